@@ -1,0 +1,157 @@
+#!/usr/bin/env python3
+"""Benchmark: two-photon luminosity grid points/s on one MI355X.
+
+Workload (one step): the full upcgen luminosity path for one configuration,
+Pb-Pb 5.02 TeV, Woods-Saxon form-factor flux (FLUX_POINT 0), XNXN breakup
+(BREAKUP_MODE 2), M 3.56-50 GeV x 1001, Y -6..6 x 121, z x 100, dimuons:
+  tables (rho0, G_AA, 10^6-knot form factor, breakup)  -> UpcCrossSection::init
+  flux rows + QAGS form-factor flux + L(M, Y) grid      -> prepareTwoPhotonLumi
+  sigma(z, M), sigma(Y, M), total                        -> computeNuclXsection
+  sampler CDFs (2-D (Y, M) and 1001 z rows)              -> UpcSampler1D/2D
+value = 121,121 grid points / step time.  Inputs are configuration scalars:
+everything the step reads lives in HBM before the timed region starts.
+
+Prints one JSON line.  `--gpus` above 1 is not supported (one GPU per box).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+NM, NY, NZ = 1001, 121, 100
+POINTS = NM * NY
+# Algorithmic work per (M, Y) point of the reference quadrature
+# (src/UpcCrossSection.cpp:252-267): 2*5*120*120 + 2*120*120 + 2*120 FLOP.
+W_ALG = 2 * 5 * 120 * 120 + 2 * 120 * 120 + 2 * 120
+FP64_PEAK_TFLOPS = 78.6
+BF16_PEAK_TFLOPS = 2500.0
+HBM_PEAK_GBS = 8000.0
+
+
+def workload_config(**over):
+    import upcgen_amd as U
+    kw = dict(Z=82, A=208, R=6.68, a=0.447, sqrts=5020.0, flux_point=0, breakup_mode=2,
+              use_polarized=0, nm=NM, mmin=3.56, mmax=50.0, ny=NY, ymin=-6.0, ymax=6.0,
+              nz=NZ, zmin=-1.0, zmax=1.0, proc_id=13, lep_a=0.0)
+    kw.update(over)
+    return U.Config.make(**kw)
+
+
+def cpu_baseline():
+    """The CPU oracle (C restatement of the reference's OpenMP path, -O2 -fopenmp)
+    on a bounded sample of the same workload: a 126 x 121 grid over the same M and
+    Y ranges (15,246 of the 121,121 bins), after building the same tables."""
+    import oracle as O
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    cores = max(1, min(16, cores))
+    nm_s = 126
+    t0 = time.perf_counter()
+    o = O.Oracle(flux_point=0, breakup_mode=2, nm=nm_s, ny=NY, nz=NZ, proc_id=13, num_threads=cores)
+    t_tables = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    _, nbad = o.lumi_grid()
+    t_grid = time.perf_counter() - t1
+    o.close()
+    n = nm_s * NY
+    return {"value": n / t_grid, "unit": "grid points/s", "cores": cores, "kind": "port",
+            "sample": f"{nm_s}x{NY} grid over the same M, Y ranges ({n} of {POINTS} bins), "
+                      f"OpenMP static M-row split on {cores} threads; lumi phase {t_grid:.2f} s, "
+                      f"tables (form factor + 10^6-knot breakup) {t_tables:.2f} s not included",
+            "qags_bad": nbad}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--form", default="fp64", choices=["fp64", "mfma", "surrogate"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the luminosity grid "
+              "shards by M rows with no exchange, but multi-GPU runs are not part of this build",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import upcgen_amd as U
+
+    form = {"fp64": U.FORM_FP64, "mfma": U.FORM_MFMA, "surrogate": U.FORM_SURROGATE}[args.form]
+    torch.cuda.set_device(0)
+    stream = torch.cuda.current_stream()
+    ctx = U.Context(workload_config())
+    L = torch.empty((NM, NY), dtype=torch.float64, device="cuda")
+
+    def step():
+        ctx.rebuild_tables()
+        ctx.lumi_grid(form, out=L)
+        zm = ctx.cs_zm(0)
+        cs, _, tot = ctx.nuc_cs_ym(L)
+        ctx.pdf_init(cs.reshape(1, -1))
+        ctx.pdf_init(zm)
+        return tot
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    ctx.set_timing(True)
+    ctx.timings(reset=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        tot = step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ph = ctx.timings(reset=True)
+    ctx.set_timing(False)
+    inf = ctx.info
+
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    value = POINTS / (ms_per_step / 1e3)
+    per_launch = {k: (v[0] / v[1] if v[1] else 0.0) for k, v in ph.items()}
+    dominant = max(("qags", "lumi", "tables", "flux"), key=lambda k: ph[k][0])
+    # roofline of the luminosity contraction (algorithmic FLOP of the reference quadrature)
+    lumi_ms = per_launch["lumi"]
+    lumi_tflops = W_ALG * POINTS / (lumi_ms * 1e-3) / 1e12 if lumi_ms > 0 else 0.0
+    roof = {"bound": "mfma", "kernel": "k_lumi_fp64 (luminosity contraction)",
+            "achieved": round(lumi_tflops, 4), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(lumi_tflops / FP64_PEAK_TFLOPS, 5), "traffic": None,
+            "algorithmic_flop_per_launch": W_ALG * POINTS, "avg_launch_ms": round(lumi_ms, 4)}
+    out = {
+        "metric": "lumi grid pts/s at 1001x121 XNXN: FP64 vs MFMA vs surrogate at stated rel err",
+        "value": round(value, 1),
+        "unit": "grid points/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (configuration scalars only; the grid is deterministic)",
+        "config": {"workload": "Pb-Pb 5.02 TeV, Woods-Saxon form-factor flux, XNXN (mode 2), "
+                               "1001x121 (M 3.56-50 GeV, Y -6..6), z x 100, PROC_ID 13",
+                   "form": args.form, "grid_points": POINTS, "parallelism": "single GPU"},
+        "roofline": roof,
+        "phases_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in ph.items()},
+        "dominant_phase": dominant,
+        "qags": {"calls": inf.qags_calls, "integrand_evals": inf.qags_evals, "bad": inf.qags_bad},
+        "tot_cs_mb": tot,
+    }
+    if not args.no_cpu_baseline:
+        cb = cpu_baseline()
+        out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,162 @@
+/*
+ * upc_lumi.h -- C ABI of the MI355X-native two-photon luminosity and nuclear
+ * cross-section engine (libupcgen_amd.so).
+ *
+ * Drop-in boundary for upcgen's UpcCrossSection / UpcGenerator path.  The
+ * reference exposes this path only as a C++ class API (include/UpcCrossSection.h
+ * :174-239, include/UpcGenerator.h:75-128); every entry point below names the
+ * member function it replaces.  The rebuilt C++ facade (upcgen_amd/csrc/
+ * UpcCrossSection.h, UpcGenerator.h) is a thin layer over these calls.
+ *
+ * Conventions
+ *   - plain C types only; `stream` is a hipStream_t passed as void* (NULL =
+ *     the null stream);
+ *   - pointers named d_* are device (HBM) pointers owned by the caller, h_*
+ *     are host pointers;
+ *   - every function returns 0 on success or a negative UPC_E* code; the
+ *     message of the last failure on the calling thread is upc_last_error();
+ *   - a context is one configuration (one reference process); contexts are
+ *     independent and may live on the same device;
+ *   - calls that return host values (h_*) synchronise `stream`.
+ */
+#ifndef UPC_LUMI_H
+#define UPC_LUMI_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define UPC_OK 0
+#define UPC_EINVAL (-1)   /* bad argument or configuration */
+#define UPC_EDEVICE (-2)  /* HIP runtime error */
+#define UPC_ENOMEM (-3)   /* device allocation failed */
+#define UPC_ESTATE (-4)   /* call not valid in the context's state */
+
+/* Luminosity compute forms (BASELINE "FP64 vs MFMA vs surrogate"). */
+#define UPC_FORM_FP64 0       /* reference quadrature, FP64 on the GPU */
+#define UPC_FORM_MFMA 1       /* shared-grid F.K.F^T on MFMA (split-bf16) */
+#define UPC_FORM_SURROGATE 2  /* MLP surrogate with quadrature fallback */
+
+/* Configuration: the parameters.in keys that reach the luminosity path
+ * (src/UpcGenerator.cpp:183-305; defaults of include/UpcCrossSection.h:72-164). */
+typedef struct upc_config {
+  int Z, A;              /* NUCLEUS_Z, NUCLEUS_A */
+  double R, a;           /* WS_R, WS_A [fm] */
+  double sqrts;          /* SQRTS [GeV]; g1 = g2 = sqrts / (2 m_p) */
+  int flux_point;        /* FLUX_POINT */
+  int breakup_mode;      /* BREAKUP_MODE 1..4 */
+  int use_polarized;     /* USE_POLARIZED_CS */
+  int nm; double mmin, mmax;   /* BINS_M, MMIN, MMAX */
+  int ny; double ymin, ymax;   /* BINS_Y, YMIN, YMAX */
+  int nz; double zmin, zmax;   /* BINS_Z, ZMIN, ZMAX */
+  int proc_id;           /* PROC_ID: 11, 13, 15 (dileptons) */
+  double lep_a;          /* LEP_A */
+} upc_config;
+
+/* Scalars derived at init (UpcCrossSection::init, src/UpcCrossSection.cpp:116-137). */
+typedef struct upc_info {
+  double g1, g2;         /* Lorentz factors */
+  double factor;         /* Z^2 alpha / (pi^2 hc^2) */
+  double mNucl;          /* nucleon mass */
+  double rho0;           /* Woods-Saxon normalisation (calcWSRho) */
+  double csNN;           /* sigma_NN [fm^2] used by G_AA (prepareGAA) */
+  double p20;            /* breakup probability at the 20 fm clamp (1 for mode 1) */
+  long long qags_calls;  /* form-factor flux integrals run by the last grid */
+  long long qags_bad;    /* of which ended with ier != 0 (the reference aborts) */
+  long long qags_evals;  /* integrand evaluations of the last grid */
+} upc_info;
+
+/* Phases timed with HIP events when timing is enabled (upc_ctx_set_timing). */
+#define UPC_PHASE_TABLES 0  /* init tables: rho0, G_AA, form factor, breakup */
+#define UPC_PHASE_FLUX 1    /* flux rows: geometry, point-charge flux, task lists */
+#define UPC_PHASE_QAGS 2    /* form-factor flux integrals (QAGS kernel) */
+#define UPC_PHASE_LUMI 3    /* luminosity contraction (FP64 / MFMA / surrogate) */
+#define UPC_PHASE_XSEC 4    /* elementary and nuclear cross-section tables */
+#define UPC_PHASE_PDF 5     /* sampler CDF builds */
+#define UPC_NPHASES 8
+
+typedef struct upc_timings {
+  double ms[UPC_NPHASES];       /* summed device time per phase since the last reset */
+  long long launches[UPC_NPHASES];
+} upc_timings;
+
+typedef struct upc_ctx upc_ctx;
+
+const char *upc_last_error(void);
+int upc_device_count(int *h_count);
+
+/* UpcCrossSection::init minus the luminosity grid: calcWSRho, prepareGAA,
+ * prepareFormFac, prepareBreakupProb (src/UpcCrossSection.cpp:116-131). All
+ * tables are built on the GPU. */
+int upc_ctx_create(const upc_config *cfg, void *stream, upc_ctx **out);
+int upc_ctx_destroy(upc_ctx *ctx);
+int upc_ctx_info(upc_ctx *ctx, upc_info *h_info);
+/* Rebuild the tables of an existing context in place (no allocation). */
+int upc_ctx_rebuild_tables(upc_ctx *ctx, void *stream);
+/* Event timing of the phases (no reference counterpart: replaces its
+ * progress logging, src/UpcCrossSection.cpp:554-557).  upc_ctx_timings
+ * synchronises on the recorded events and resets the sums when reset != 0. */
+int upc_ctx_set_timing(upc_ctx *ctx, int enable);
+int upc_ctx_timings(upc_ctx *ctx, upc_timings *h_out, int reset);
+
+/* ---- per-point functions of the path (batched; device arrays of n) ---- */
+/* UpcCrossSection::fluxPoint(b, k)            src/UpcCrossSection.cpp:166-178 */
+int upc_flux_point(upc_ctx *ctx, const double *d_b, const double *d_k, int n, double *d_out,
+                   void *stream);
+/* UpcCrossSection::fluxForm(b, k)             src/UpcCrossSection.cpp:194-218
+ * d_ier (may be NULL) receives the QAGS status per point. */
+int upc_flux_form(upc_ctx *ctx, const double *d_b, const double *d_k, int n, double *d_out,
+                  int *d_ier, void *stream);
+/* UpcCrossSection::calcFormFac(Q2) (tabulated = 0) or its spline (1)
+ *                                              src/UpcCrossSection.cpp:436-461 */
+int upc_form_factor(upc_ctx *ctx, const double *d_q2, int n, int tabulated, double *d_out,
+                    void *stream);
+/* UpcCrossSection::calcBreakupProb(b, mode) (tabulated = 0) or the
+ * prepareBreakupProb spline (1, b <= 20)       src/UpcCrossSection.cpp:416-434, 752-1019 */
+int upc_breakup_prob(upc_ctx *ctx, const double *d_b, int n, int mode, int tabulated,
+                     double *d_out, void *stream);
+/* G_AA spline (which = 0) or T_A spline (which = 1) of prepareGAA
+ *                                              src/UpcCrossSection.cpp:364-414 */
+int upc_nuclear_overlap(upc_ctx *ctx, const double *d_b, int n, int which, double *d_out,
+                        void *stream);
+/* calcTwoPhotonLumi(M, Y) / calcTwoPhotonLumiPol(ns, np, M, Y) at n
+ * arbitrary points                              src/UpcCrossSection.cpp:221-335
+ * Unpolarised -> d_L; polarised -> d_Ls, d_Lp. */
+int upc_lumi_points(upc_ctx *ctx, const double *d_M, const double *d_Y, int n, double *d_L,
+                    double *d_Ls, double *d_Lp, void *stream);
+
+/* ---- grids ---- */
+/* prepareTwoPhotonLumi's grid: [nm][ny], lower bin edges, x dm x dy
+ *                                              src/UpcCrossSection.cpp:494-576 */
+int upc_lumi_grid(upc_ctx *ctx, int form, double *d_L, double *d_Ls, double *d_Lp, void *stream);
+/* fillCrossSectionZM(cs, zmin, zmax, nz, mmin, mmax, nm, flag): [nm][nz]
+ *                                              src/UpcCrossSection.cpp:337-362 */
+int upc_cs_zm(upc_ctx *ctx, int flag, double *d_zm, void *stream);
+/* Elementary sigma(M) (flag 0), sigma_S(M) (1), sigma_PS(M) (2) at nm lower edges
+ *                                              src/UpcTwoPhotonDilep.cpp:46-121 */
+int upc_cs_m(upc_ctx *ctx, int flag, double *d_cs, void *stream);
+/* calcNucCrossSectionYM(cs, ratio, totCS): cs/ratio [ny][nm]; *h_tot_cs is
+ * incremented by the total in mb like the reference   src/UpcCrossSection.cpp:594-698 */
+int upc_nuc_cs_ym(upc_ctx *ctx, const double *d_L, const double *d_Ls, const double *d_Lp,
+                  double *d_cs, double *d_ratio, double *h_tot_cs, void *stream);
+
+/* ---- sampler tables (include/UpcSampler.h:37-142) ---- */
+/* gsl_histogram_pdf_init on `nrows` independent rows of `n` bins:
+ * d_sum[r*(n+1) + i].  Returns UPC_EINVAL if any bin is negative (GSL_EDOM). */
+int upc_pdf_init(const double *d_bins, int nrows, int n, double *d_sum, void *stream);
+/* Draw n events (Y, M, z) from the generator's tables like
+ * UpcGenerator::generateEvent (src/UpcGenerator.cpp:735-757): the 2-D (Y,M)
+ * pdf d_sum_ym [(ny*nm)+1], the per-M z pdfs d_sum_z [nm][nz+1] (polarised:
+ * d_sum_zs and d_sum_zp with d_ratio [ny][nm]).  Counter-based Philox4x32-10,
+ * stream (seed, offset). */
+int upc_sample_events(upc_ctx *ctx, const double *d_sum_ym, const double *d_sum_z,
+                      const double *d_sum_zs, const double *d_sum_zp, const double *d_ratio,
+                      unsigned long long seed, unsigned long long offset, long long n,
+                      double *d_y, double *d_m, double *d_z, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* UPC_LUMI_H */

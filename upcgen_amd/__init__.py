@@ -1,0 +1,292 @@
+"""upcgen_amd: MI355X-native two-photon luminosity / nuclear cross-section engine.
+
+Python bindings of the C ABI in include/upc_lumi.h (libupcgen_amd.so, HIP
+kernels for gfx950).  torch provides device buffers and streams; every
+computation runs in the HIP library.  There is no CPU fallback: loading fails
+loudly when the library or a GPU is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+__all__ = ["Config", "Info", "Context", "load", "FORM_FP64", "FORM_MFMA", "FORM_SURROGATE",
+           "UpcError", "LIB_PATH"]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libupcgen_amd.so")
+
+FORM_FP64, FORM_MFMA, FORM_SURROGATE = 0, 1, 2
+
+
+class UpcError(RuntimeError):
+    pass
+
+
+class Config(C.Structure):
+    """upc_config: the parameters.in keys that reach the luminosity path."""
+    _fields_ = [
+        ("Z", C.c_int), ("A", C.c_int),
+        ("R", C.c_double), ("a", C.c_double), ("sqrts", C.c_double),
+        ("flux_point", C.c_int), ("breakup_mode", C.c_int), ("use_polarized", C.c_int),
+        ("nm", C.c_int), ("mmin", C.c_double), ("mmax", C.c_double),
+        ("ny", C.c_int), ("ymin", C.c_double), ("ymax", C.c_double),
+        ("nz", C.c_int), ("zmin", C.c_double), ("zmax", C.c_double),
+        ("proc_id", C.c_int), ("lep_a", C.c_double),
+    ]
+
+    # include/UpcCrossSection.h:72-164 defaults, PROC_ID 13
+    DEFAULTS = dict(Z=82, A=208, R=6.68, a=0.447, sqrts=5020.0, flux_point=1, breakup_mode=1,
+                    use_polarized=0, nm=1001, mmin=3.56, mmax=50.0, ny=121, ymin=-6.0, ymax=6.0,
+                    nz=100, zmin=-1.0, zmax=1.0, proc_id=13, lep_a=0.0)
+
+    @classmethod
+    def make(cls, **kw) -> "Config":
+        v = dict(cls.DEFAULTS)
+        unknown = set(kw) - set(v)
+        if unknown:
+            raise TypeError(f"unknown config keys: {sorted(unknown)}")
+        v.update(kw)
+        return cls(**v)
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class Info(C.Structure):
+    _fields_ = [("g1", C.c_double), ("g2", C.c_double), ("factor", C.c_double),
+                ("mNucl", C.c_double), ("rho0", C.c_double), ("csNN", C.c_double),
+                ("p20", C.c_double), ("qags_calls", C.c_longlong), ("qags_bad", C.c_longlong),
+                ("qags_evals", C.c_longlong)]
+
+
+class Timings(C.Structure):
+    _fields_ = [("ms", C.c_double * 8), ("launches", C.c_longlong * 8)]
+
+PHASES = ("tables", "flux", "qags", "lumi", "xsec", "pdf")
+
+
+# Every symbol declared in include/upc_lumi.h with its ctypes signature.
+_vp, _i, _d, _ll, _ull = C.c_void_p, C.c_int, C.c_double, C.c_longlong, C.c_ulonglong
+SIGNATURES = {
+    "upc_last_error": (C.c_char_p, []),
+    "upc_device_count": (_i, [C.POINTER(_i)]),
+    "upc_ctx_create": (_i, [C.POINTER(Config), _vp, C.POINTER(_vp)]),
+    "upc_ctx_destroy": (_i, [_vp]),
+    "upc_ctx_info": (_i, [_vp, C.POINTER(Info)]),
+    "upc_ctx_rebuild_tables": (_i, [_vp, _vp]),
+    "upc_ctx_set_timing": (_i, [_vp, _i]),
+    "upc_ctx_timings": (_i, [_vp, C.POINTER(Timings), _i]),
+    "upc_flux_point": (_i, [_vp, _vp, _vp, _i, _vp, _vp]),
+    "upc_flux_form": (_i, [_vp, _vp, _vp, _i, _vp, _vp, _vp]),
+    "upc_form_factor": (_i, [_vp, _vp, _i, _i, _vp, _vp]),
+    "upc_breakup_prob": (_i, [_vp, _vp, _i, _i, _i, _vp, _vp]),
+    "upc_nuclear_overlap": (_i, [_vp, _vp, _i, _i, _vp, _vp]),
+    "upc_lumi_points": (_i, [_vp, _vp, _vp, _i, _vp, _vp, _vp, _vp]),
+    "upc_lumi_grid": (_i, [_vp, _i, _vp, _vp, _vp, _vp]),
+    "upc_cs_zm": (_i, [_vp, _i, _vp, _vp]),
+    "upc_cs_m": (_i, [_vp, _i, _vp, _vp]),
+    "upc_nuc_cs_ym": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, C.POINTER(_d), _vp]),
+    "upc_pdf_init": (_i, [_vp, _i, _i, _vp, _vp]),
+    "upc_sample_events": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _ull, _ull, _ll, _vp, _vp, _vp, _vp]),
+}
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load libupcgen_amd.so (build it first with upcgen_amd.build)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise UpcError(f"{LIB_PATH} is missing: run `python -m upcgen_amd.build`")
+        lib = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def _check(rc: int, what: str):
+    if rc != 0:
+        raise UpcError(f"{what} failed ({rc}): {load().upc_last_error().decode()}")
+
+
+def _dptr(t):
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise UpcError("device tensor expected")
+    if not t.is_contiguous():
+        raise UpcError("contiguous tensor expected")
+    return C.c_void_p(t.data_ptr())
+
+
+def _stream(stream):
+    if stream is None:
+        import torch
+        stream = torch.cuda.current_stream()
+    return C.c_void_p(stream.cuda_stream if hasattr(stream, "cuda_stream") else int(stream))
+
+
+class Context:
+    """One configuration = one upc_ctx (UpcCrossSection after init())."""
+
+    def __init__(self, config: Config | None = None, stream=None, **kw):
+        import torch
+        if not torch.cuda.is_available():
+            raise UpcError("no HIP device visible: upcgen_amd runs on the GPU only")
+        self.config = config if config is not None else Config.make(**kw)
+        self._ctx = C.c_void_p()
+        _check(load().upc_ctx_create(C.byref(self.config), _stream(stream), C.byref(self._ctx)),
+               "upc_ctx_create")
+
+    def close(self):
+        if self._ctx:
+            load().upc_ctx_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def info(self) -> Info:
+        inf = Info()
+        _check(load().upc_ctx_info(self._ctx, C.byref(inf)), "upc_ctx_info")
+        return inf
+
+    def set_timing(self, enable=True):
+        _check(load().upc_ctx_set_timing(self._ctx, int(enable)), "upc_ctx_set_timing")
+
+    def timings(self, reset=True):
+        """Summed device milliseconds per phase (HIP events on the launch stream)."""
+        t = Timings()
+        _check(load().upc_ctx_timings(self._ctx, C.byref(t), int(reset)), "upc_ctx_timings")
+        return {name: (t.ms[i], t.launches[i]) for i, name in enumerate(PHASES)}
+
+    def rebuild_tables(self, stream=None):
+        _check(load().upc_ctx_rebuild_tables(self._ctx, _stream(stream)), "upc_ctx_rebuild_tables")
+
+    # --- batched per-point functions (torch float64 cuda tensors) ---
+    def _empty(self, n, dtype=None):
+        import torch
+        return torch.empty(n, dtype=dtype or torch.float64, device="cuda")
+
+    def flux_point(self, b, k, stream=None):
+        out = self._empty(b.numel())
+        _check(load().upc_flux_point(self._ctx, _dptr(b), _dptr(k), b.numel(), _dptr(out),
+                                     _stream(stream)), "upc_flux_point")
+        return out
+
+    def flux_form(self, b, k, stream=None):
+        import torch
+        out = self._empty(b.numel())
+        ier = self._empty(b.numel(), torch.int32)
+        _check(load().upc_flux_form(self._ctx, _dptr(b), _dptr(k), b.numel(), _dptr(out), _dptr(ier),
+                                    _stream(stream)), "upc_flux_form")
+        return out, ier
+
+    def form_factor(self, q2, tabulated=False, stream=None):
+        out = self._empty(q2.numel())
+        _check(load().upc_form_factor(self._ctx, _dptr(q2), q2.numel(), int(tabulated), _dptr(out),
+                                      _stream(stream)), "upc_form_factor")
+        return out
+
+    def breakup_prob(self, b, mode=None, tabulated=False, stream=None):
+        out = self._empty(b.numel())
+        m = self.config.breakup_mode if mode is None else mode
+        _check(load().upc_breakup_prob(self._ctx, _dptr(b), b.numel(), m, int(tabulated), _dptr(out),
+                                       _stream(stream)), "upc_breakup_prob")
+        return out
+
+    def nuclear_overlap(self, b, which=0, stream=None):
+        out = self._empty(b.numel())
+        _check(load().upc_nuclear_overlap(self._ctx, _dptr(b), b.numel(), which, _dptr(out),
+                                          _stream(stream)), "upc_nuclear_overlap")
+        return out
+
+    def lumi_points(self, M, Y, stream=None):
+        n = M.numel()
+        if self.config.use_polarized:
+            ls, lp = self._empty(n), self._empty(n)
+            _check(load().upc_lumi_points(self._ctx, _dptr(M), _dptr(Y), n, None, _dptr(ls), _dptr(lp),
+                                          _stream(stream)), "upc_lumi_points")
+            return ls, lp
+        out = self._empty(n)
+        _check(load().upc_lumi_points(self._ctx, _dptr(M), _dptr(Y), n, _dptr(out), None, None,
+                                      _stream(stream)), "upc_lumi_points")
+        return out
+
+    # --- grids ---
+    def lumi_grid(self, form=FORM_FP64, out=None, stream=None):
+        """prepareTwoPhotonLumi grid [nm][ny] (x dm x dy). Polarised -> (Ls, Lp)."""
+        import torch
+        nm, ny = self.config.nm, self.config.ny
+        if self.config.use_polarized:
+            ls, lp = out if out is not None else (
+                torch.empty((nm, ny), dtype=torch.float64, device="cuda"),
+                torch.empty((nm, ny), dtype=torch.float64, device="cuda"))
+            _check(load().upc_lumi_grid(self._ctx, form, None, _dptr(ls), _dptr(lp), _stream(stream)),
+                   "upc_lumi_grid")
+            return ls, lp
+        L = out if out is not None else torch.empty((nm, ny), dtype=torch.float64, device="cuda")
+        _check(load().upc_lumi_grid(self._ctx, form, _dptr(L), None, None, _stream(stream)),
+               "upc_lumi_grid")
+        return L
+
+    def cs_zm(self, flag=0, stream=None):
+        import torch
+        out = torch.empty((self.config.nm, self.config.nz), dtype=torch.float64, device="cuda")
+        _check(load().upc_cs_zm(self._ctx, flag, _dptr(out), _stream(stream)), "upc_cs_zm")
+        return out
+
+    def cs_m(self, flag=0, stream=None):
+        out = self._empty(self.config.nm)
+        _check(load().upc_cs_m(self._ctx, flag, _dptr(out), _stream(stream)), "upc_cs_m")
+        return out
+
+    def nuc_cs_ym(self, lumi, stream=None, tot_cs=0.0):
+        """calcNucCrossSectionYM: returns (cs [ny][nm], ratio or None, totCS [mb])."""
+        import torch
+        nm, ny = self.config.nm, self.config.ny
+        cs = torch.empty((ny, nm), dtype=torch.float64, device="cuda")
+        tot = C.c_double(tot_cs)
+        if self.config.use_polarized:
+            ls, lp = lumi
+            ratio = torch.empty((ny, nm), dtype=torch.float64, device="cuda")
+            _check(load().upc_nuc_cs_ym(self._ctx, None, _dptr(ls), _dptr(lp), _dptr(cs), _dptr(ratio),
+                                        C.byref(tot), _stream(stream)), "upc_nuc_cs_ym")
+            return cs, ratio, tot.value
+        _check(load().upc_nuc_cs_ym(self._ctx, _dptr(lumi), None, None, _dptr(cs), None, C.byref(tot),
+                                    _stream(stream)), "upc_nuc_cs_ym")
+        return cs, None, tot.value
+
+    @staticmethod
+    def pdf_init(bins, stream=None):
+        """gsl_histogram_pdf_init per row of a 2-D tensor (rows, n) -> (rows, n+1)."""
+        import torch
+        b2 = bins.reshape(-1, bins.shape[-1]) if bins.dim() > 1 else bins.reshape(1, -1)
+        rows, n = b2.shape
+        out = torch.empty((rows, n + 1), dtype=torch.float64, device="cuda")
+        _check(load().upc_pdf_init(_dptr(b2.contiguous()), rows, n, _dptr(out), _stream(stream)),
+               "upc_pdf_init")
+        return out
+
+    def sample_events(self, sum_ym, sum_z=None, sum_zs=None, sum_zp=None, ratio=None, n=1000,
+                      seed=12345, offset=0, stream=None):
+        y, m, z = self._empty(n), self._empty(n), self._empty(n)
+        _check(load().upc_sample_events(self._ctx, _dptr(sum_ym), _dptr(sum_z), _dptr(sum_zs),
+                                        _dptr(sum_zp), _dptr(ratio), seed, offset, n, _dptr(y), _dptr(m),
+                                        _dptr(z), _stream(stream)), "upc_sample_events")
+        return y, m, z

@@ -1,0 +1,67 @@
+"""Build libupcgen_amd.so (HIP kernels for gfx950 + C ABI + C++ facade) in-tree.
+
+Usage: python -m upcgen_amd.build   (or upcgen_amd.build.build()).
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+ROOT = os.path.dirname(HERE)
+BUILD = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libupcgen_amd.so")
+EXE = os.path.join(HERE, "upcgen")
+
+HIP_SOURCES = ["upc_capi.hip", "upc_tables.hip", "upc_flux_lumi.hip", "upc_xsec.hip", "upc_mfma.hip",
+               "upc_surrogate.hip"]
+CXX_SOURCES = ["UpcCrossSection.cpp", "UpcGenerator.cpp"]
+
+ARCH = os.environ.get("UPC_OFFLOAD_ARCH", "gfx950")
+HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+COMMON = ["-O3", "-fPIC", "-std=c++17", f"-I{os.path.join(ROOT, 'include')}", f"-I{CSRC}"]
+# Packed-FP32 VALU forms are disabled for every HIP translation unit.
+HIPFLAGS = COMMON + [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+                     "-munsafe-fp-atomics", "-Wno-unused-command-line-argument"]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"command failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    return r
+
+
+def _compile(src: str) -> str:
+    path = os.path.join(CSRC, src)
+    obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
+    flags = HIPFLAGS if src.endswith(".hip") else COMMON + ["-x", "c++"]
+    extra = ["-x", "hip"] if src.endswith(".hip") else []
+    _run([HIPCC] + flags + extra + ["-c", path, "-o", obj])
+    return obj
+
+
+def build(verbose: bool = False) -> str:
+    os.makedirs(BUILD, exist_ok=True)
+    sources = HIP_SOURCES + [s for s in CXX_SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    with cf.ThreadPoolExecutor(max_workers=min(8, len(sources))) as ex:
+        objs = list(ex.map(_compile, sources))
+    tmp = LIB + ".tmp"
+    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp] + objs)
+    os.replace(tmp, LIB)
+    main = os.path.join(CSRC, "main.cpp")
+    if os.path.exists(main):
+        _run([HIPCC] + COMMON + ["-x", "c++", main, "-o", EXE, f"-L{HERE}", "-lupcgen_amd",
+                                 f"-Wl,-rpath,{HERE}", "-Wl,-rpath,$ORIGIN"])
+    if verbose:
+        print(f"built {LIB}")
+    return LIB
+
+
+if __name__ == "__main__":
+    build(verbose=True)
+    sys.exit(0)
