@@ -46,8 +46,23 @@ def test_tables_match_oracle(gpu_ctx, oracle_ctx):
     assert np.max(np.abs(gaa - og)) < 1e-13
     assert rel(ta, ot) < 1e-12
     q2 = np.concatenate([10 ** rng.uniform(-9, np.log10(1.99), 500), [1e-9, 1.999998]])
-    assert rel(_np(g.form_factor(_t(q2), False)), [o.formfac(x) for x in q2]) < 1e-11
-    assert rel(_np(g.form_factor(_t(q2), True)), [o.formfac_spline(x) for x in q2]) < 1e-11
+    # calcFormFac cancels like 1/(QR)^2 as Q -> 0: one-ulp differences of sin,
+    # cos, sinh, cosh between libraries grow to ~1e-15/(QR)^2 relative there
+    qr2 = q2 / 0.1973269718 ** 2 * 6.68 ** 2
+    tol = 5e-14 + 2e-15 / qr2
+    ff_an = _np(g.form_factor(_t(q2), False))
+    ref_an = np.array([o.formfac(x) for x in q2])
+    assert np.all(np.abs(ff_an - ref_an) <= tol * 208)
+    ff_sp = _np(g.form_factor(_t(q2), True))
+    ref_sp = np.array([o.formfac_spline(x) for x in q2])
+    # the spline carries the noise of its interval's knots (2e-6 apart): the
+    # whole first interval interpolates from the knot at Q2 = 1e-9
+    dq2 = (2.0 - 1e-9) / 1e6
+    q2k = 1e-9 + np.floor((q2 - 1e-9) / dq2) * dq2
+    tol_sp = 20 * (5e-14 + 2e-15 / (q2k / 0.1973269718 ** 2 * 6.68 ** 2))
+    assert np.all(np.abs(ff_sp - ref_sp) <= tol_sp * 208)
+    print(f"form factor max abs diff: analytic {np.max(np.abs(ff_an - ref_an)):.2e}, "
+          f"spline {np.max(np.abs(ff_sp - ref_sp)):.2e}")
     bb = np.concatenate([10 ** rng.uniform(-5, np.log10(20), 300), [20.0, 1e-6]])
     for mode in (2, 3, 4):
         assert rel(_np(g.breakup_prob(_t(bb), mode)), [o.breakup_prob(x, mode) for x in bb]) < 1e-12
@@ -167,7 +182,7 @@ def test_cross_section_tables(gpu_ctx, oracle_ctx):
         if pol:
             cs, ratio, tot = g.nuc_cs_ym(lumi)
             ocs, oratio, otot = o.nuc_cs_ym(lumi_s=_np(lumi[0]), lumi_p=_np(lumi[1]))
-            assert rel(_np(ratio), oratio) < 1e-13
+            assert rel(_np(ratio), oratio) < 1e-12
         else:
             cs, _, tot = g.nuc_cs_ym(lumi)
             ocs, _, otot = o.nuc_cs_ym(lumi=_np(lumi))
