@@ -131,12 +131,17 @@ int upc_lumi_points(upc_ctx *ctx, const double *d_M, const double *d_Y, int n, d
 /* prepareTwoPhotonLumi's grid: [nm][ny], lower bin edges, x dm x dy
  *                                              src/UpcCrossSection.cpp:494-576 */
 int upc_lumi_grid(upc_ctx *ctx, int form, double *d_L, double *d_Ls, double *d_Lp, void *stream);
-/* fillCrossSectionZM(cs, zmin, zmax, nz, mmin, mmax, nm, flag): [nm][nz]
- *                                              src/UpcCrossSection.cpp:337-362 */
-int upc_cs_zm(upc_ctx *ctx, int flag, double *d_zm, void *stream);
+/* fillCrossSectionZM(cs, zmin, zmax, nz, mmin, mmax, nm, flag): d_zm [nm][nz]
+ * at lower bin edges, x hc^2 1e7 / dm      src/UpcCrossSection.cpp:337-362 */
+int upc_cs_zm(upc_ctx *ctx, double zmin, double zmax, int nz, double mmin, double mmax, int nm, int flag,
+              double *d_zm, void *stream);
 /* Elementary sigma(M) (flag 0), sigma_S(M) (1), sigma_PS(M) (2) at nm lower edges
  *                                              src/UpcTwoPhotonDilep.cpp:46-121 */
 int upc_cs_m(upc_ctx *ctx, int flag, double *d_cs, void *stream);
+/* UpcTwoPhotonDilep::calcCrossSectionM/MPolS/MPolPS (flag 0/1/2; d_z ignored)
+ * or calcCrossSectionZM/ZMPolS/ZMPolPS (flag 3/4/5) at n points  src/UpcTwoPhotonDilep.cpp:46-134 */
+int upc_elem_cs(upc_ctx *ctx, int flag, const double *d_m, const double *d_z, int n, double *d_out,
+                void *stream);
 /* calcNucCrossSectionYM(cs, ratio, totCS): cs/ratio [ny][nm]; *h_tot_cs is
  * incremented by the total in mb like the reference   src/UpcCrossSection.cpp:594-698 */
 int upc_nuc_cs_ym(upc_ctx *ctx, const double *d_L, const double *d_Ls, const double *d_Lp,
@@ -155,6 +160,34 @@ int upc_sample_events(upc_ctx *ctx, const double *d_sum_ym, const double *d_sum_
                       const double *d_sum_zs, const double *d_sum_zp, const double *d_ratio,
                       unsigned long long seed, unsigned long long offset, long long n,
                       double *d_y, double *d_m, double *d_z, void *stream);
+
+/* ---- event generation for the dilepton processes ---- */
+/* Kinematic options of UpcGenerator (parameters.in keys NON_ZERO_GAM_PT,
+ * DO_PT_CUT, PT_MIN, DO_ETA_CUT, ETA_MIN, ETA_MAX). */
+typedef struct upc_event_opts {
+  int nonzero_gam_pt;
+  int do_pt_cut;
+  double pt_min;
+  int do_eta_cut;
+  double eta_min, eta_max;
+} upc_event_opts;
+
+/* Photon transverse-momentum sampling tables of getPhotonPt, built from the
+ * context's form-factor table for Lorentz factor gtot
+ *                                              src/UpcCrossSection.cpp:1021-1051 */
+int upc_photon_pt_tables(upc_ctx *ctx, double gtot, void *stream);
+/* n getPhotonPt(E) draws at photon energies d_E (stream seed). */
+int upc_photon_pt(upc_ctx *ctx, const double *d_E, long long n, unsigned long long seed, double *d_out,
+                  void *stream);
+/* n pair events as UpcGenerator::generateEvent (src/UpcGenerator.cpp:715-790):
+ * (Y, M) and z as upc_sample_events, pair momentum (getPairMomentum), lepton
+ * 4-momenta (pairProduction) and checkKinCuts.  d_p [n][2][4] = (px, py, pz,
+ * E) per lepton, d_pdg [n][2], d_pass[n] = 1 accepted, 2 rejected by cuts,
+ * 0 not drawn (random number beyond the last CDF entry). */
+int upc_generate_pairs(upc_ctx *ctx, const double *d_sum_ym, const double *d_sum_z, const double *d_sum_zs,
+                       const double *d_sum_zp, const double *d_ratio, const upc_event_opts *opts,
+                       unsigned long long seed, unsigned long long offset, long long n, double *d_p, int *d_pdg,
+                       int *d_pass, void *stream);
 
 #ifdef __cplusplus
 }

@@ -94,6 +94,7 @@ def lib():
         L.orc_dilep_cs_zm.argtypes = [i, d, d, d]
         L.orc_fill_cs_zm.argtypes = [vp, _DP, i]
         L.orc_nuc_cs_ym.argtypes = [vp, _DP, _DP, _DP, _DP, _DP, _DP]
+        L.orc_photon_pt_hist.argtypes = [vp, d, d, _DP]
         L.orc_pdf_init.restype = i
         L.orc_pdf_init.argtypes = [_DP, i, _DP]
         L.orc_pdf_sample.restype = d
@@ -170,6 +171,11 @@ class Oracle:
     def cs_zm(self, flag=0):
         out = np.zeros((self.params.nm, self.params.nz))
         lib().orc_fill_cs_zm(self._ctx, _ptr(out), flag)
+        return out
+
+    def photon_pt_hist(self, E, gtot):
+        out = np.zeros(5000)
+        lib().orc_photon_pt_hist(self._ctx, E, gtot, _ptr(out))
         return out
 
     def nuc_cs_ym(self, lumi=None, lumi_s=None, lumi_p=None):

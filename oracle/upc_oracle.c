@@ -1320,6 +1320,21 @@ void orc_nuc_cs_ym(const orc_ctx *c, const double *lumi, const double *lumiS, co
   *tot_cs *= 1e-6;
 }
 
+/* getPhotonPt's histogram (src/UpcCrossSection.cpp:1021-1051): bin contents
+ * of the 5000-bin pt distribution over (0, 6 hc / R] at photon energy E. */
+void orc_photon_pt_hist(const orc_ctx *c, double E, double gtot, double *out)
+{
+  const double pi2x4 = 4 * M_PI * M_PI;
+  const double ereds = (E * E) / (gtot * gtot);
+  const int nbins = 5000;
+  for (int bin = 1; bin <= nbins; bin++) {
+    const double pt = 6. * kHc / c->p.R / nbins * bin;
+    const double arg = pt * pt + ereds;
+    const double f = orc_spline_eval(&c->ff, arg, NULL);
+    out[bin - 1] = (f * f) * pt * pt * pt / (pi2x4 * arg * arg);
+  }
+}
+
 /* ------------------------------------------------------------------------ */
 /* Histogram pdf (include/UpcSampler.h:53-71, 101-121)                       */
 /* ------------------------------------------------------------------------ */

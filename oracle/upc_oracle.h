@@ -95,6 +95,9 @@ void orc_fill_cs_zm(const orc_ctx *c, double *out, int flag);
 void orc_nuc_cs_ym(const orc_ctx *c, const double *lumi, const double *lumiS,
                    const double *lumiP, double *cs, double *ratio, double *tot_cs);
 
+/* getPhotonPt histogram contents (5000 bins over (0, 6 hc/R]) at energy E. */
+void orc_photon_pt_hist(const orc_ctx *c, double E, double gtot, double *out);
+
 /* gsl_histogram_pdf_init semantics: sum[0..n] from n non-negative bins.
  * Returns 0, or -1 if a bin is negative (GSL_EDOM). */
 int orc_pdf_init(const double *bins, int n, double *sum);

@@ -60,6 +60,11 @@ class Info(C.Structure):
                 ("qags_evals", C.c_longlong)]
 
 
+class EventOpts(C.Structure):
+    _fields_ = [("nonzero_gam_pt", C.c_int), ("do_pt_cut", C.c_int), ("pt_min", C.c_double),
+                ("do_eta_cut", C.c_int), ("eta_min", C.c_double), ("eta_max", C.c_double)]
+
+
 class Timings(C.Structure):
     _fields_ = [("ms", C.c_double * 8), ("launches", C.c_longlong * 8)]
 
@@ -84,10 +89,15 @@ SIGNATURES = {
     "upc_nuclear_overlap": (_i, [_vp, _vp, _i, _i, _vp, _vp]),
     "upc_lumi_points": (_i, [_vp, _vp, _vp, _i, _vp, _vp, _vp, _vp]),
     "upc_lumi_grid": (_i, [_vp, _i, _vp, _vp, _vp, _vp]),
-    "upc_cs_zm": (_i, [_vp, _i, _vp, _vp]),
+    "upc_cs_zm": (_i, [_vp, _d, _d, _i, _d, _d, _i, _i, _vp, _vp]),
     "upc_cs_m": (_i, [_vp, _i, _vp, _vp]),
+    "upc_elem_cs": (_i, [_vp, _i, _vp, _vp, _i, _vp, _vp]),
     "upc_nuc_cs_ym": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, C.POINTER(_d), _vp]),
     "upc_pdf_init": (_i, [_vp, _i, _i, _vp, _vp]),
+    "upc_photon_pt_tables": (_i, [_vp, _d, _vp]),
+    "upc_photon_pt": (_i, [_vp, _vp, _ll, _ull, _vp, _vp]),
+    "upc_generate_pairs": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, C.POINTER(EventOpts), _ull, _ull, _ll, _vp,
+                                _vp, _vp, _vp]),
     "upc_sample_events": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _ull, _ull, _ll, _vp, _vp, _vp, _vp]),
 }
 
@@ -245,15 +255,31 @@ class Context:
                "upc_lumi_grid")
         return L
 
-    def cs_zm(self, flag=0, stream=None):
+    def cs_zm(self, flag=0, stream=None, zmin=None, zmax=None, nz=None, mmin=None, mmax=None, nm=None):
+        """fillCrossSectionZM over the configured grid (or the ranges given): [nm][nz]."""
         import torch
-        out = torch.empty((self.config.nm, self.config.nz), dtype=torch.float64, device="cuda")
-        _check(load().upc_cs_zm(self._ctx, flag, _dptr(out), _stream(stream)), "upc_cs_zm")
+        c = self.config
+        zmin = c.zmin if zmin is None else zmin
+        zmax = c.zmax if zmax is None else zmax
+        nz = c.nz if nz is None else nz
+        mmin = c.mmin if mmin is None else mmin
+        mmax = c.mmax if mmax is None else mmax
+        nm = c.nm if nm is None else nm
+        out = torch.empty((nm, nz), dtype=torch.float64, device="cuda")
+        _check(load().upc_cs_zm(self._ctx, zmin, zmax, nz, mmin, mmax, nm, flag, _dptr(out),
+                                _stream(stream)), "upc_cs_zm")
         return out
 
     def cs_m(self, flag=0, stream=None):
         out = self._empty(self.config.nm)
         _check(load().upc_cs_m(self._ctx, flag, _dptr(out), _stream(stream)), "upc_cs_m")
+        return out
+
+    def elem_cs(self, flag, m, z=None, stream=None):
+        """Elementary sigma: flag 0/1/2 = sigma(M), S, PS; 3/4/5 = dsigma/dz(z, M), S, PS."""
+        out = self._empty(m.numel())
+        _check(load().upc_elem_cs(self._ctx, flag, _dptr(m), _dptr(z), m.numel(), _dptr(out),
+                                  _stream(stream)), "upc_elem_cs")
         return out
 
     def nuc_cs_ym(self, lumi, stream=None, tot_cs=0.0):
@@ -282,6 +308,28 @@ class Context:
         _check(load().upc_pdf_init(_dptr(b2.contiguous()), rows, n, _dptr(out), _stream(stream)),
                "upc_pdf_init")
         return out
+
+    def photon_pt_tables(self, gtot, stream=None):
+        _check(load().upc_photon_pt_tables(self._ctx, gtot, _stream(stream)), "upc_photon_pt_tables")
+
+    def photon_pt(self, E, seed=1, stream=None):
+        out = self._empty(E.numel())
+        _check(load().upc_photon_pt(self._ctx, _dptr(E), E.numel(), seed, _dptr(out), _stream(stream)),
+               "upc_photon_pt")
+        return out
+
+    def generate_pairs(self, sum_ym, sum_z=None, sum_zs=None, sum_zp=None, ratio=None, n=1000, seed=12345,
+                       offset=0, opts=None, stream=None):
+        """Dilepton events: (p [n,2,4] (px,py,pz,E), pdg [n,2], pass [n])."""
+        import torch
+        o = opts if opts is not None else EventOpts(0, 0, 0.0, 0, 0.0, 0.0)
+        p = torch.empty((n, 2, 4), dtype=torch.float64, device="cuda")
+        pdg = torch.empty((n, 2), dtype=torch.int32, device="cuda")
+        ok = torch.empty(n, dtype=torch.int32, device="cuda")
+        _check(load().upc_generate_pairs(self._ctx, _dptr(sum_ym), _dptr(sum_z), _dptr(sum_zs), _dptr(sum_zp),
+                                         _dptr(ratio), C.byref(o), seed, offset, n, _dptr(p), _dptr(pdg),
+                                         _dptr(ok), _stream(stream)), "upc_generate_pairs")
+        return p, pdg, ok
 
     def sample_events(self, sum_ym, sum_z=None, sum_zs=None, sum_zp=None, ratio=None, n=1000,
                       seed=12345, offset=0, stream=None):

@@ -18,7 +18,7 @@ LIB = os.path.join(HERE, "libupcgen_amd.so")
 EXE = os.path.join(HERE, "upcgen")
 
 HIP_SOURCES = ["upc_capi.hip", "upc_tables.hip", "upc_flux_lumi.hip", "upc_xsec.hip", "upc_mfma.hip",
-               "upc_surrogate.hip"]
+               "upc_surrogate.hip", "upc_events.hip"]
 CXX_SOURCES = ["UpcCrossSection.cpp", "UpcGenerator.cpp"]
 
 ARCH = os.environ.get("UPC_OFFLOAD_ARCH", "gfx950")
@@ -39,8 +39,9 @@ def _run(cmd):
 def _compile(src: str) -> str:
     path = os.path.join(CSRC, src)
     obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
-    flags = HIPFLAGS if src.endswith(".hip") else COMMON + ["-x", "c++"]
-    extra = ["-x", "hip"] if src.endswith(".hip") else []
+    # host-only C++ of the facade is compiled as HIP too: it calls the HIP runtime
+    flags = HIPFLAGS
+    extra = ["-x", "hip"]
     _run([HIPCC] + flags + extra + ["-c", path, "-o", obj])
     return obj
 
@@ -55,8 +56,8 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     main = os.path.join(CSRC, "main.cpp")
     if os.path.exists(main):
-        _run([HIPCC] + COMMON + ["-x", "c++", main, "-o", EXE, f"-L{HERE}", "-lupcgen_amd",
-                                 f"-Wl,-rpath,{HERE}", "-Wl,-rpath,$ORIGIN"])
+        _run([HIPCC] + HIPFLAGS + ["-x", "hip", main, "-o", EXE, f"-L{HERE}", "-lupcgen_amd",
+                                   "-Wl,-rpath,$ORIGIN"])
     if verbose:
         print(f"built {LIB}")
     return LIB

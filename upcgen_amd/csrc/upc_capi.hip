@@ -125,6 +125,7 @@ int upc_ctx_destroy(upc_ctx* ctx)
   free_tables(ctx);
   free_mfma(ctx);
   free_surrogate(ctx);
+  free_events(ctx);
   delete ctx;
   return 0;
 }
@@ -246,16 +247,25 @@ int upc_lumi_grid(upc_ctx* c, int form, double* d_L, double* d_Ls, double* d_Lp,
   }
 }
 
-int upc_cs_zm(upc_ctx* c, int flag, double* d_zm, void* stream)
+int upc_cs_zm(upc_ctx* c, double zmin, double zmax, int nz, double mmin, double mmax, int nm, int flag,
+              double* d_zm, void* stream)
 {
-  if (!c || !d_zm || flag < 0 || flag > 2) return set_error("bad arguments"), UPC_EINVAL;
-  return launch_cs_zm(c, flag, d_zm, S(stream));
+  if (!c || !d_zm || flag < 0 || flag > 2 || nz <= 0 || nm <= 0 || !(zmax > zmin) || !(mmax > mmin))
+    return set_error("bad arguments"), UPC_EINVAL;
+  return launch_cs_zm(c, zmin, zmax, nz, mmin, mmax, nm, flag, d_zm, S(stream));
 }
 
 int upc_cs_m(upc_ctx* c, int flag, double* d_cs, void* stream)
 {
   if (!c || !d_cs || flag < 0 || flag > 2) return set_error("bad arguments"), UPC_EINVAL;
   return launch_cs_m(c, flag, d_cs, S(stream));
+}
+
+int upc_elem_cs(upc_ctx* c, int flag, const double* d_m, const double* d_z, int n, double* d_out, void* stream)
+{
+  if (!c || n < 0 || flag < 0 || flag > 5 || (n && (!d_m || !d_out || (flag >= 3 && !d_z))))
+    return set_error("bad arguments"), UPC_EINVAL;
+  return launch_elem_cs(c, flag, d_m, d_z, n, d_out, S(stream));
 }
 
 int upc_nuc_cs_ym(upc_ctx* c, const double* d_L, const double* d_Ls, const double* d_Lp, double* d_cs,
@@ -280,6 +290,29 @@ int upc_sample_events(upc_ctx* c, const double* d_sum_ym, const double* d_sum_z,
   if (!c || n < 0 || (n && (!d_sum_ym || !d_y || !d_m || !d_z))) return set_error("bad arguments"), UPC_EINVAL;
   return launch_sample_events(c, d_sum_ym, d_sum_z, d_sum_zs, d_sum_zp, d_ratio, seed, offset, n, d_y, d_m, d_z,
                               S(stream));
+}
+
+int upc_photon_pt_tables(upc_ctx* c, double gtot, void* stream)
+{
+  if (!c || !(gtot >= 1.0)) return set_error("bad arguments"), UPC_EINVAL;
+  return build_pt_tables(c, gtot, S(stream));
+}
+
+int upc_photon_pt(upc_ctx* c, const double* d_E, long long n, unsigned long long seed, double* d_out, void* stream)
+{
+  if (!c || n < 0 || (n && (!d_E || !d_out))) return set_error("bad arguments"), UPC_EINVAL;
+  return launch_photon_pt(c, d_E, n, seed, d_out, S(stream));
+}
+
+int upc_generate_pairs(upc_ctx* c, const double* d_sum_ym, const double* d_sum_z, const double* d_sum_zs,
+                       const double* d_sum_zp, const double* d_ratio, const upc_event_opts* opts,
+                       unsigned long long seed, unsigned long long offset, long long n, double* d_p, int* d_pdg,
+                       int* d_pass, void* stream)
+{
+  if (!c || !opts || n < 0 || (n && (!d_sum_ym || !d_p || !d_pdg || !d_pass)))
+    return set_error("bad arguments"), UPC_EINVAL;
+  return launch_generate_pairs(c, d_sum_ym, d_sum_z, d_sum_zs, d_sum_zp, d_ratio, opts, seed, offset, n, d_p, d_pdg,
+                               d_pass, S(stream));
 }
 
 }  // extern "C"

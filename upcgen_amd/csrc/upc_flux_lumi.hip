@@ -4,10 +4,14 @@
 //   calcTwoPhotonLumi / calcTwoPhotonLumiPol    src/UpcCrossSection.cpp:221-335
 //   prepareTwoPhotonLumi grid loop              src/UpcCrossSection.cpp:494-576
 //
-// Layout: every (M, Y) point owns two flux rows (k1 = M/2 e^Y with g1, k2 =
-// M/2 e^-Y with g2), each the 120 flux values N(b_j, k) on the point's own
-// log b grid.  Row r = 2*point + side, flux[r*120 + j].  Nodes with b <= 2R
-// (form-factor flux) become QAGS tasks; the rest take the point-charge flux.
+// Layout: a flux row is the 120 values N(b_j, k) on the log b grid of photon
+// energy k.  Point p (of npts) uses row p for k1 = M/2 e^Y (g1) and a second
+// row for k2 = M/2 e^-Y (g2).  On a grid whose Y range is symmetric (ymin =
+// -ymax, lower bin edges) the k2 of bin (im, iy >= 1) is the k1 of its mirror
+// bin (im, ny - iy) up to one rounding of y, so only the iy = 0 column gets
+// own k2 rows: rows [npts, npts + nm).  Otherwise rows [npts, 2 npts) hold the
+// k2 rows.  flux[r*120 + j].  Nodes with b <= 2R (form-factor flux) become
+// QAGS tasks; the rest take the point-charge flux.
 #include "upc_internal.h"
 #include "upc_qags.h"
 
@@ -22,7 +26,16 @@ constexpr int kQagsBigCap = 1000;
 struct FluxParams {
   double bmin, R, g1, g2, factor, twoR;
   int A, isPoint;
+  int mirror_ny;  // > 0: symmetric grid with ny Y bins (k2 rows shared with mirror bins)
 };
+
+// Row holding k2 of point p.
+__device__ __forceinline__ int k2_row(int p, int npts, int mirror_ny)
+{
+  if (mirror_ny <= 0) return npts + p;
+  const int im = p / mirror_ny, iy = p - im * mirror_ny;
+  return iy ? im * mirror_ny + (mirror_ny - iy) : npts + im;
+}
 
 __device__ __forceinline__ double flux_point(double b, double k, double g1, double factor)
 {
@@ -71,12 +84,19 @@ __global__ void k_grid_points(int nm, double mmin, double mmax, int ny, double y
 }
 
 // Row geometry and the number of form-factor (QAGS) nodes of each row.
-__global__ void k_rows(const double* __restrict__ pts, int npts, FluxParams fp,
+__global__ void k_rows(const double* __restrict__ pts, int npts, int nrows, FluxParams fp,
                        double* __restrict__ row_k, double* __restrict__ row_logd, int* __restrict__ row_cnt)
 {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= 2 * npts) return;
-  const int p = r >> 1, side = r & 1;
+  if (r >= nrows) return;
+  int p, side;
+  if (r < npts) {
+    p = r;
+    side = 0;
+  } else {
+    p = fp.mirror_ny > 0 ? (r - npts) * fp.mirror_ny : r - npts;
+    side = 1;
+  }
   const double M = pts[4 * p], Y = pts[4 * p + 1];
   const double k = side ? M / 2. * exp(-Y) : M / 2. * exp(Y);
   const double g = side ? fp.g2 : fp.g1;
@@ -179,7 +199,7 @@ struct PointTask {
 };
 
 template <int CAP, class TS>
-__global__ __launch_bounds__(256) void k_flux_qags(TS ts, const unsigned long long* __restrict__ ntasks_p,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_flux_qags(TS ts, const unsigned long long* __restrict__ ntasks_p,
                                                    unsigned long long* __restrict__ cursor,
                                                    FluxParams fp, const double4* __restrict__ ff,
                                                    double* __restrict__ flux, int* __restrict__ ier_out,
@@ -368,9 +388,10 @@ __global__ __launch_bounds__(64 * kLumiWaves) void k_lumi_fp64(
     M = pts[4 * p];
     dm = pts[4 * p + 2];
     dy = pts[4 * p + 3];
-    const double logd1 = row_logd[2 * p], logd2 = row_logd[2 * p + 1];
-    const double* f1 = flux + (long)(2 * p) * kNB1;
-    const double* f2 = flux + (long)(2 * p + 1) * kNB1;
+    const int r2 = k2_row(p, npts, fp.mirror_ny);
+    const double logd1 = row_logd[p], logd2 = row_logd[r2];
+    const double* f1 = flux + (long)p * kNB1;
+    const double* f2 = flux + (long)r2 * kNB1;
     for (int j = lane; j < kNB1; j += 64) {
       double bl, bh;
       const double b1 = row_node(fp.bmin, logd1, j, &bl, &bh);
@@ -465,9 +486,10 @@ __global__ __launch_bounds__(64 * kLumiWaves) void k_lumi_fp64(
 // ---------------------------------------------------------------------------
 static inline int nblk(long n, int bs) { return (int)((n + bs - 1) / bs); }
 
-static FluxParams flux_params(const upc_ctx* c)
+static FluxParams flux_params(const upc_ctx* c, int mirror_ny = 0)
 {
   FluxParams fp;
+  fp.mirror_ny = mirror_ny;
   fp.R = c->cfg.R;
   fp.bmin = c->cfg.flux_point ? c->cfg.R : 0.05 * c->cfg.R;
   fp.g1 = c->g1;
@@ -533,16 +555,23 @@ static int big_ws(int n, Qags<kQagsBigCap>** out)
   return 0;
 }
 
+static int grid_mirror_ny(const upc_ctx* c, bool grid)
+{
+  return grid && c->cfg.ymin == -c->cfg.ymax ? c->cfg.ny : 0;
+}
+
+static int flux_rows(int npts, int mirror_ny) { return mirror_ny > 0 ? npts + npts / mirror_ny : 2 * npts; }
+
 // Runs flux rows + QAGS for the points already in c->work.d_points.
-static int run_flux(upc_ctx* c, int npts, hipStream_t s)
+static int run_flux(upc_ctx* c, int npts, int mirror_ny, hipStream_t s)
 {
   LumiWork& w = c->work;
-  const FluxParams fp = flux_params(c);
-  const int nrows = 2 * npts;
+  const FluxParams fp = flux_params(c, mirror_ny);
+  const int nrows = flux_rows(npts, mirror_ny);
   {
   PhaseScope ph(c, s, UPC_PHASE_FLUX);
   UPC_HIP(hipMemsetAsync(w.d_counters, 0, sizeof(unsigned long long) * 8, s));
-  hipLaunchKernelGGL(k_rows, dim3(nblk(nrows, 256)), dim3(256), 0, s, w.d_points, npts, fp, w.d_row_k,
+  hipLaunchKernelGGL(k_rows, dim3(nblk(nrows, 256)), dim3(256), 0, s, w.d_points, npts, nrows, fp, w.d_row_k,
                      w.d_row_logd, w.d_row_cnt);
   hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, s, w.d_row_cnt, nrows, w.d_row_off,
                      w.d_counters + 4);
@@ -611,9 +640,10 @@ int lumi_points_fp64(upc_ctx* c, const double* d_pts, int npts, double* d_L, dou
     hipLaunchKernelGGL(k_copy_points, dim3(nblk(npts, 256)), dim3(256), 0, s, d_pts, d_pts + npts, npts,
                        c->work.d_points);
   }
-  rc = run_flux(c, npts, s);
+  const int mirror_ny = grid_mirror_ny(c, d_pts == nullptr);
+  rc = run_flux(c, npts, mirror_ny, s);
   if (rc) return rc;
-  const FluxParams fp = flux_params(c);
+  const FluxParams fp = flux_params(c, mirror_ny);
   const dim3 grid(nblk(npts, kLumiWaves)), block(64 * kLumiWaves);
   {
   PhaseScope ph(c, s, UPC_PHASE_LUMI);

@@ -45,6 +45,16 @@ struct LumiWork {
   size_t points_cap = 0;
 };
 
+// upc_events.hip
+int build_pt_tables(upc_ctx* c, double gtot, hipStream_t s);
+void free_events(upc_ctx* c);
+int launch_photon_pt(upc_ctx* c, const double* d_E, long long n, unsigned long long seed, double* d_out,
+                     hipStream_t s);
+int launch_generate_pairs(upc_ctx* c, const double* d_sum_ym, const double* d_sum_z, const double* d_sum_zs,
+                          const double* d_sum_zp, const double* d_ratio, const upc_event_opts* opts,
+                          unsigned long long seed, unsigned long long offset, long long n, double* d_p, int* d_pdg,
+                          int* d_pass, hipStream_t s);
+
 }  // namespace upc
 
 namespace upc {
@@ -52,6 +62,16 @@ struct TimedRec {
   int phase;
   hipEvent_t beg, end;
 };
+// upc_events.hip
+int build_pt_tables(upc_ctx* c, double gtot, hipStream_t s);
+void free_events(upc_ctx* c);
+int launch_photon_pt(upc_ctx* c, const double* d_E, long long n, unsigned long long seed, double* d_out,
+                     hipStream_t s);
+int launch_generate_pairs(upc_ctx* c, const double* d_sum_ym, const double* d_sum_z, const double* d_sum_zs,
+                          const double* d_sum_zp, const double* d_ratio, const upc_event_opts* opts,
+                          unsigned long long seed, unsigned long long offset, long long n, double* d_p, int* d_pdg,
+                          int* d_pass, hipStream_t s);
+
 }  // namespace upc
 
 struct upc_ctx {
@@ -81,6 +101,7 @@ struct upc_ctx {
   // MFMA / surrogate state is attached by their translation units.
   void* mfma = nullptr;
   void* surrogate = nullptr;
+  void* events = nullptr;
 };
 
 namespace upc {
@@ -124,8 +145,11 @@ int launch_flux_form(upc_ctx* c, const double* d_b, const double* d_k, int n, do
 void free_work(upc_ctx* c);
 
 // upc_xsec.hip
-int launch_cs_zm(upc_ctx* c, int flag, double* d_zm, hipStream_t s);
+int launch_cs_zm(upc_ctx* c, double zmin, double zmax, int nz, double mmin, double mmax, int nm, int flag,
+                 double* d_zm, hipStream_t s);
 int launch_cs_m(upc_ctx* c, int flag, double* d_cs, hipStream_t s);
+int launch_elem_cs(upc_ctx* c, int flag, const double* d_m, const double* d_z, int n, double* d_out,
+                   hipStream_t s);
 int launch_nuc_cs_ym(upc_ctx* c, const double* d_L, const double* d_Ls, const double* d_Lp,
                      double* d_cs, double* d_ratio, double* h_tot, hipStream_t s);
 int launch_pdf_init(const double* d_bins, int nrows, int n, double* d_sum, hipStream_t s);
@@ -133,5 +157,15 @@ int launch_sample_events(upc_ctx* c, const double* d_sum_ym, const double* d_sum
                          const double* d_sum_zs, const double* d_sum_zp, const double* d_ratio,
                          unsigned long long seed, unsigned long long offset, long long n,
                          double* d_y, double* d_m, double* d_z, hipStream_t s);
+
+// upc_events.hip
+int build_pt_tables(upc_ctx* c, double gtot, hipStream_t s);
+void free_events(upc_ctx* c);
+int launch_photon_pt(upc_ctx* c, const double* d_E, long long n, unsigned long long seed, double* d_out,
+                     hipStream_t s);
+int launch_generate_pairs(upc_ctx* c, const double* d_sum_ym, const double* d_sum_z, const double* d_sum_zs,
+                          const double* d_sum_zp, const double* d_ratio, const upc_event_opts* opts,
+                          unsigned long long seed, unsigned long long offset, long long n, double* d_p, int* d_pdg,
+                          int* d_pass, hipStream_t s);
 
 }  // namespace upc

@@ -6,6 +6,7 @@
 //   UpcSampler1D/2D (gsl_histogram*_pdf)        include/UpcSampler.h:37-142
 //   UpcGenerator::generateEvent (Y, M, z draw)  src/UpcGenerator.cpp:735-757
 #include "upc_internal.h"
+#include "upc_philox.h"
 
 namespace upc {
 
@@ -132,6 +133,24 @@ __global__ void k_cs_m(int flag, int pdg, double aLep, int nm, double mmin, doub
   out[im] = flag == 0 ? dilep_cs_m(mp, aLep, m) : (flag == 1 ? dilep_cs_m_pol_s(mp, m) : dilep_cs_m_pol_ps(mp, m));
 }
 
+__global__ void k_elem_cs(int flag, int pdg, double aLep, const double* __restrict__ m, const double* __restrict__ z,
+                          int n, double* __restrict__ out)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double mp = lep_mass(pdg);
+  double v = 0;
+  switch (flag) {
+    case 0: v = dilep_cs_m(mp, aLep, m[i]); break;
+    case 1: v = dilep_cs_m_pol_s(mp, m[i]); break;
+    case 2: v = dilep_cs_m_pol_ps(mp, m[i]); break;
+    case 3: v = dilep_cs_zm(mp, aLep, z[i], m[i]); break;
+    case 4: v = dilep_cs_zm_pol_s(mp, z[i], m[i]); break;
+    default: v = dilep_cs_zm_pol_ps(mp, z[i], m[i]); break;
+  }
+  out[i] = v;
+}
+
 // calcNucCrossSectionYM: cs[iy][im] from L[im][iy]; per-block partial totals.
 template <bool POL>
 __global__ __launch_bounds__(256) void k_nuc_cs_ym(int pdg, double aLep, int nm, double mmin, double mmax, int ny,
@@ -232,49 +251,6 @@ __global__ __launch_bounds__(256) void k_pdf_init(const double* __restrict__ bin
   }
 }
 
-// ---------------------------------------------------------------------------
-// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).
-// ---------------------------------------------------------------------------
-struct Philox {
-  uint32_t c[4];
-  uint32_t k[2];
-  __device__ __forceinline__ void round_()
-  {
-    constexpr uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
-    const uint64_t p0 = (uint64_t)M0 * c[0];
-    const uint64_t p1 = (uint64_t)M1 * c[2];
-    const uint32_t h0 = (uint32_t)(p0 >> 32), l0 = (uint32_t)p0;
-    const uint32_t h1 = (uint32_t)(p1 >> 32), l1 = (uint32_t)p1;
-    const uint32_t n0 = h1 ^ c[1] ^ k[0];
-    const uint32_t n2 = h0 ^ c[3] ^ k[1];
-    c[0] = n0;
-    c[1] = l1;
-    c[2] = n2;
-    c[3] = l0;
-  }
-  __device__ __forceinline__ void gen()
-  {
-    constexpr uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
-    uint32_t k0 = k[0], k1 = k[1];
-#pragma unroll
-    for (int r = 0; r < 10; ++r) {
-      if (r) {
-        k[0] += W0;
-        k[1] += W1;
-      }
-      round_();
-    }
-    k[0] = k0;
-    k[1] = k1;
-  }
-};
-
-// 53-bit uniform in [0, 1) from two 32-bit words.
-__device__ __forceinline__ double u01(uint32_t hi, uint32_t lo)
-{
-  return ((double)(hi >> 5) * 67108864.0 + (double)(lo >> 6)) * (1.0 / 9007199254740992.0);
-}
-
 // gsl_histogram_pdf_sample on a prefix table: sum[i] <= r < sum[i+1].
 __device__ __forceinline__ long pdf_find(const double* __restrict__ sum, long n, double r)
 {
@@ -300,24 +276,14 @@ __global__ void k_sample_events(const double* __restrict__ sum_ym, const double*
 {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  Philox ph;
+  Philox4x32 ph;
   const unsigned long long ctr = offset + (unsigned long long)i;
-  ph.c[0] = (uint32_t)ctr;
-  ph.c[1] = (uint32_t)(ctr >> 32);
-  ph.c[2] = 0;
-  ph.c[3] = 0;
-  ph.k[0] = (uint32_t)seed;
-  ph.k[1] = (uint32_t)(seed >> 32);
-  ph.gen();
-  double r1 = u01(ph.c[0], ph.c[1]);
-  double r2 = u01(ph.c[2], ph.c[3]);
-  ph.c[0] = (uint32_t)ctr;
-  ph.c[1] = (uint32_t)(ctr >> 32);
-  ph.c[2] = 1;
-  ph.c[3] = 0;
-  ph.gen();
-  double r3 = u01(ph.c[0], ph.c[1]);
-  const double r4 = u01(ph.c[2], ph.c[3]);
+  ph.block(seed, ctr, 0);
+  double r1 = philox_u01(ph.c[0], ph.c[1]);
+  double r2 = philox_u01(ph.c[2], ph.c[3]);
+  ph.block(seed, ctr, 1);
+  double r3 = philox_u01(ph.c[0], ph.c[1]);
+  const double r4 = philox_u01(ph.c[2], ph.c[3]);
   const long nym = (long)ny * nm;
   if (r1 == 1.0) r1 = 0.0;
   if (r2 == 1.0) r2 = 0.0;
@@ -362,12 +328,22 @@ __global__ void k_sample_events(const double* __restrict__ sum_ym, const double*
 // ---------------------------------------------------------------------------
 static inline int nblk(long n, int bs) { return (int)((n + bs - 1) / bs); }
 
-int launch_cs_zm(upc_ctx* c, int flag, double* d_zm, hipStream_t s)
+int launch_cs_zm(upc_ctx* c, double zmin, double zmax, int nz, double mmin, double mmax, int nm, int flag,
+                 double* d_zm, hipStream_t s)
 {
   const upc_config& g = c->cfg;
   PhaseScope ph(c, s, UPC_PHASE_XSEC);
-  hipLaunchKernelGGL(k_cs_zm, dim3(nblk((long)g.nm * g.nz, 256)), dim3(256), 0, s, flag, g.proc_id, g.lep_a, g.nm,
-                     g.mmin, g.mmax, g.nz, g.zmin, g.zmax, d_zm);
+  hipLaunchKernelGGL(k_cs_zm, dim3(nblk((long)nm * nz, 256)), dim3(256), 0, s, flag, g.proc_id, g.lep_a, nm, mmin,
+                     mmax, nz, zmin, zmax, d_zm);
+  UPC_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_elem_cs(upc_ctx* c, int flag, const double* d_m, const double* d_z, int n, double* d_out, hipStream_t s)
+{
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k_elem_cs, dim3(nblk(n, 256)), dim3(256), 0, s, flag, c->cfg.proc_id, c->cfg.lep_a, d_m, d_z, n,
+                     d_out);
   UPC_HIP(hipGetLastError());
   return 0;
 }
