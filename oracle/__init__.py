@@ -95,6 +95,7 @@ def lib():
         L.orc_fill_cs_zm.argtypes = [vp, _DP, i]
         L.orc_nuc_cs_ym.argtypes = [vp, _DP, _DP, _DP, _DP, _DP, _DP]
         L.orc_photon_pt_hist.argtypes = [vp, d, d, _DP]
+        L.orc_set_qags_tol.argtypes = [vp, d, d]
         L.orc_pdf_init.restype = i
         L.orc_pdf_init.argtypes = [_DP, i, _DP]
         L.orc_pdf_sample.restype = d
@@ -172,6 +173,9 @@ class Oracle:
         out = np.zeros((self.params.nm, self.params.nz))
         lib().orc_fill_cs_zm(self._ctx, _ptr(out), flag)
         return out
+
+    def set_qags_tol(self, epsabs, epsrel):
+        lib().orc_set_qags_tol(self._ctx, epsabs, epsrel)
 
     def photon_pt_hist(self, E, gtot):
         out = np.zeros(5000)

@@ -674,6 +674,7 @@ int orc_qags(orc_fn f, void *p, double a, double b, double epsabs, double epsrel
 /* ------------------------------------------------------------------------ */
 struct orc_ctx {
   orc_params p;
+  double qags_epsabs, qags_epsrel; /* src/UpcCrossSection.cpp:209: 1e-4, 1e-4 */
   double g1, g2, factor, mNucl, rho0, csNN;
   orc_spline ta, gaa, ff, breakp;
   int have_breakp;
@@ -926,6 +927,8 @@ orc_ctx *orc_create(const orc_params *p)
 {
   orc_ctx *c = (orc_ctx *)calloc(1, sizeof(orc_ctx));
   c->p = *p;
+  c->qags_epsabs = 1e-4;
+  c->qags_epsrel = 1e-4;
   /* g1 = g2 = sqrts / (2 m_p), src/UpcGenerator.cpp:188-192 */
   c->g1 = p->sqrts / (2. * kMProt);
   c->g2 = p->sqrts / (2. * kMProt);
@@ -948,6 +951,15 @@ void orc_destroy(orc_ctx *c)
   orc_spline_free(&c->ff);
   if (c->have_breakp) orc_spline_free(&c->breakp);
   free(c);
+}
+
+/* Study hook: the quadrature tolerances of the form-factor flux (the
+ * reference uses 1e-4, 1e-4); used to measure how much of L is quadrature
+ * error. */
+void orc_set_qags_tol(orc_ctx *c, double epsabs, double epsrel)
+{
+  c->qags_epsabs = epsabs;
+  c->qags_epsrel = epsrel;
 }
 
 double orc_rho0(const orc_ctx *c) { return c->rho0; }
@@ -996,7 +1008,8 @@ double orc_flux_form(const orc_ctx *c, double b, double k, int *neval, int *ier)
   ff_par fp = {c, b, k, c->g1, 0};
   double res, err;
   int nev = 0, last = 0;
-  const int e = orc_qags(flux_form_integrand, &fp, 0., 10., 1e-4, 1e-4, 1000, &res, &err, &nev, &last);
+  const int e = orc_qags(flux_form_integrand, &fp, 0., 10., c->qags_epsabs, c->qags_epsrel, 1000, &res, &err,
+                         &nev, &last);
   if (neval) *neval = nev;
   if (ier) *ier = e ? e : (fp.err ? 100 : 0);
   const double Q = res / c->p.A;

@@ -61,6 +61,7 @@ void orc_spline_free(orc_spline *s);
 orc_ctx *orc_create(const orc_params *p);   /* rho0, G_AA, FF and breakup tables */
 void orc_destroy(orc_ctx *c);
 
+void orc_set_qags_tol(orc_ctx *c, double epsabs, double epsrel);
 double orc_rho0(const orc_ctx *c);
 double orc_factor(const orc_ctx *c);
 double orc_g1(const orc_ctx *c);

@@ -87,9 +87,9 @@ int main(int argc, char** argv)
   UPC_LOG_INFO << "Starting generation process...";
   upcGenerator->generateEvents();
   char buf[64];
-  std::snprintf(buf, sizeof(buf), "%.9g", upcGenerator->totNuclX());
+  std::snprintf(buf, sizeof(buf), "%.17g", upcGenerator->totNuclX());
   UPC_LOG_INFO << "totCS_mb " << buf;
-  std::snprintf(buf, sizeof(buf), "%.9g", upcGenerator->fidNuclX());
+  std::snprintf(buf, sizeof(buf), "%.17g", upcGenerator->fidNuclX());
   UPC_LOG_INFO << "fidCS_mb " << buf;
   delete upcGenerator;
   return 0;
