@@ -29,6 +29,9 @@ POINTS = NM * NY
 W_ALG = 2 * 5 * 120 * 120 + 2 * 120 * 120 + 2 * 120
 FP64_PEAK_TFLOPS = 78.6
 BF16_PEAK_TFLOPS = 2500.0
+FP32_MFMA_PEAK_TFLOPS = 157.3
+# one MFMA tile step of the MFMA forms: 16 points x 16 columns x 32 deep
+KSTEP_FLOP = 2 * 16 * 16 * 32
 HBM_PEAK_GBS = 8000.0
 
 
@@ -72,7 +75,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--form", default="fp64", choices=["fp64", "mfma", "surrogate"])
+    ap.add_argument("--form", default="fp64", choices=["fp64", "mfma", "mfma_fp32", "surrogate"])
+    ap.add_argument("--no-compare", action="store_true",
+                    help="skip timing the other forms' grids and their error against the FP64 form")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -84,7 +89,9 @@ def main():
     import torch
     import upcgen_amd as U
 
-    form = {"fp64": U.FORM_FP64, "mfma": U.FORM_MFMA, "surrogate": U.FORM_SURROGATE}[args.form]
+    forms = {"fp64": U.FORM_FP64, "mfma": U.FORM_MFMA, "mfma_fp32": U.FORM_MFMA_FP32,
+             "surrogate": U.FORM_SURROGATE}
+    form = forms[args.form]
     torch.cuda.set_device(0)
     stream = torch.cuda.current_stream()
     ctx = U.Context(workload_config())
@@ -117,14 +124,49 @@ def main():
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     value = POINTS / (ms_per_step / 1e3)
     per_launch = {k: (v[0] / v[1] if v[1] else 0.0) for k, v in ph.items()}
-    dominant = max(("qags", "lumi", "tables", "flux"), key=lambda k: ph[k][0])
-    # roofline of the luminosity contraction (algorithmic FLOP of the reference quadrature)
+    dominant = max(("qags", "lumi", "tables", "flux", "project"), key=lambda k: ph[k][0])
     lumi_ms = per_launch["lumi"]
-    lumi_tflops = W_ALG * POINTS / (lumi_ms * 1e-3) / 1e12 if lumi_ms > 0 else 0.0
-    roof = {"bound": "mfma", "kernel": "k_lumi_fp64 (luminosity contraction)",
-            "achieved": round(lumi_tflops, 4), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": round(lumi_tflops / FP64_PEAK_TFLOPS, 5), "traffic": None,
-            "algorithmic_flop_per_launch": W_ALG * POINTS, "avg_launch_ms": round(lumi_ms, 4)}
+    if form == U.FORM_FP64:
+        # algorithmic FLOP of the reference quadrature per launch
+        flop = W_ALG * POINTS
+        tflops = flop / (lumi_ms * 1e-3) / 1e12 if lumi_ms > 0 else 0.0
+        roof = {"bound": "mfma", "kernel": "k_lumi_fp64 (luminosity contraction, FP64 VALU)",
+                "achieved": round(tflops, 4), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(tflops / FP64_PEAK_TFLOPS, 5), "traffic": None,
+                "algorithmic_flop_per_launch": flop, "avg_launch_ms": round(lumi_ms, 4)}
+    else:
+        # MFMA FLOP issued per launch: tile steps x 16x16x32 x products per step
+        # (split-bf16: hi*hi + hi*lo + lo*hi; FP32: one) for each luminosity (1 or 2)
+        split = form == U.FORM_MFMA
+        nprod = 3 if split else 1
+        flop = inf.mfma_ksteps * KSTEP_FLOP * nprod
+        peak = BF16_PEAK_TFLOPS if split else FP32_MFMA_PEAK_TFLOPS
+        tflops = flop / (lumi_ms * 1e-3) / 1e12 if lumi_ms > 0 else 0.0
+        roof = {"bound": "mfma", "kernel": f"k_mfma_lumi ({'split-bf16' if split else 'fp32'} banded GEMM)",
+                "achieved": round(tflops, 3), "peak": peak, "unit": "TFLOP/s",
+                "frac": round(tflops / peak, 5), "traffic": None, "mfma_flop_per_launch": flop,
+                "ksteps_per_launch": inf.mfma_ksteps, "avg_launch_ms": round(lumi_ms, 4),
+                "shared_grid_spacing_fm": inf.mfma_spacing}
+    compare = None
+    if not args.no_compare:
+        compare = {}
+        ref = torch.empty_like(L)
+        ctx.lumi_grid(U.FORM_FP64, out=ref)
+        for name, f in forms.items():
+            if f == U.FORM_SURROGATE:
+                continue
+            ctx.lumi_grid(f, out=L)  # warm
+            torch.cuda.synchronize()
+            reps = 3
+            t2 = time.perf_counter()
+            for _ in range(reps):
+                ctx.lumi_grid(f, out=L)
+            torch.cuda.synchronize()
+            ms = (time.perf_counter() - t2) * 1e3 / reps
+            err = ((L - ref).abs() / ref.abs()).flatten()
+            compare[name] = {"ms_per_grid": round(ms, 3), "grid_points_per_s": round(POINTS / ms * 1e3, 1),
+                             "max_rel_err_vs_fp64": float(err.max()),
+                             "median_rel_err_vs_fp64": float(err.median())}
     out = {
         "metric": "lumi grid pts/s at 1001x121 XNXN: FP64 vs MFMA vs surrogate at stated rel err",
         "value": round(value, 1),
@@ -144,6 +186,7 @@ def main():
         "roofline": roof,
         "phases_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in ph.items()},
         "dominant_phase": dominant,
+        "forms": compare,
         "qags": {"calls": inf.qags_calls, "integrand_evals": inf.qags_evals, "bad": inf.qags_bad},
         "tot_cs_mb": tot,
     }

@@ -34,10 +34,11 @@ extern "C" {
 #define UPC_ENOMEM (-3)   /* device allocation failed */
 #define UPC_ESTATE (-4)   /* call not valid in the context's state */
 
-/* Luminosity compute forms (BASELINE "FP64 vs MFMA vs surrogate"). */
+/* Luminosity compute forms. */
 #define UPC_FORM_FP64 0       /* reference quadrature, FP64 on the GPU */
-#define UPC_FORM_MFMA 1       /* shared-grid F.K.F^T on MFMA (split-bf16) */
+#define UPC_FORM_MFMA 1       /* FP64 flux rows, shared-grid W.D.W^T on MFMA (split-bf16, FP32 acc) */
 #define UPC_FORM_SURROGATE 2  /* MLP surrogate with quadrature fallback */
+#define UPC_FORM_MFMA_FP32 3  /* as UPC_FORM_MFMA with FP32 MFMA operands */
 
 /* Configuration: the parameters.in keys that reach the luminosity path
  * (src/UpcGenerator.cpp:183-305; defaults of include/UpcCrossSection.h:72-164). */
@@ -66,6 +67,8 @@ typedef struct upc_info {
   long long qags_calls;  /* form-factor flux integrals run by the last grid */
   long long qags_bad;    /* of which ended with ier != 0 (the reference aborts) */
   long long qags_evals;  /* integrand evaluations of the last grid */
+  long long mfma_ksteps; /* 16-point x 16-column x 32-deep MFMA tile steps of the last MFMA grid */
+  double mfma_spacing;   /* shared b-grid spacing of the MFMA forms [fm] */
 } upc_info;
 
 /* Phases timed with HIP events when timing is enabled (upc_ctx_set_timing). */
@@ -75,6 +78,7 @@ typedef struct upc_info {
 #define UPC_PHASE_LUMI 3    /* luminosity contraction (FP64 / MFMA / surrogate) */
 #define UPC_PHASE_XSEC 4    /* elementary and nuclear cross-section tables */
 #define UPC_PHASE_PDF 5     /* sampler CDF builds */
+#define UPC_PHASE_PROJECT 6 /* MFMA forms: kernel matrix on the shared grid + flux-row projection */
 #define UPC_NPHASES 8
 
 typedef struct upc_timings {
@@ -99,6 +103,10 @@ int upc_ctx_rebuild_tables(upc_ctx *ctx, void *stream);
  * progress logging, src/UpcCrossSection.cpp:554-557).  upc_ctx_timings
  * synchronises on the recorded events and resets the sums when reset != 0. */
 int upc_ctx_set_timing(upc_ctx *ctx, int enable);
+/* Node spacing h [fm] of the shared b grid of the MFMA forms (default 0.2;
+ * accepted range [0.05, 2]).  No reference counterpart: the reference has
+ * only the per-bin node quadrature. */
+int upc_ctx_set_mfma_spacing(upc_ctx *ctx, double h_fm);
 int upc_ctx_timings(upc_ctx *ctx, upc_timings *h_out, int reset);
 
 /* ---- per-point functions of the path (batched; device arrays of n) ---- */

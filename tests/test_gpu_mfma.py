@@ -1,0 +1,121 @@
+"""MFMA forms of the luminosity grid (split-bf16 and FP32 MFMA) against the
+CPU oracle and the FP64 GPU form.
+
+Bar: relative error <= 1e-5 per (M, Y) bin (MFMA_TOL).  The flux rows are the
+FP64 rows of the plain form, so the error is the shared-grid interpolation of
+the b1/b2/phi kernel plus the MFMA operand rounding; both are reported.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import upcgen_amd as U
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+pytestmark = pytest.mark.gpu
+
+MFMA_TOL = 1e-5
+FORMS = [U.FORM_MFMA, U.FORM_MFMA_FP32]
+NAMES = {U.FORM_MFMA: "split-bf16", U.FORM_MFMA_FP32: "fp32"}
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def relerr(a, b):
+    a, b = np.asarray(a, dtype=np.float64), np.asarray(b, dtype=np.float64)
+    return np.abs(a - b) / np.maximum(np.abs(b), 1e-300)
+
+
+def _check(name, got, ref, tol=MFMA_TOL):
+    e = relerr(got, ref)
+    print(f"{name}: max {e.max():.2e} median {np.median(e):.2e} p99 {np.quantile(e, 0.99):.2e}")
+    assert e.max() <= tol, (name, float(e.max()))
+    return e
+
+
+@pytest.mark.parametrize("form", FORMS)
+def test_mfma_fixtures(gpu_ctx, form):
+    gold = np.load(os.path.join(GOLD, "oracle_grids.npz"))
+    g = gpu_ctx(flux_point=1, breakup_mode=1, nm=101, ny=41, nz=50, proc_id=13)
+    _check(f"{NAMES[form]} cfg1 101x41", _np(g.lumi_grid(form)), gold["cfg1_point_mode1"])
+    assert g.info.mfma_ksteps > 0
+    g2 = gpu_ctx(flux_point=0, breakup_mode=2, nm=7, ny=5)
+    _check(f"{NAMES[form]} ff mode2 7x5", _np(g2.lumi_grid(form)), gold["ff_mode2_7x5"])
+    g3 = gpu_ctx(flux_point=0, breakup_mode=3, use_polarized=1, nm=7, ny=5)
+    ls, lp = g3.lumi_grid(form)
+    _check(f"{NAMES[form]} ff mode3 pol s", _np(ls), gold["ff_mode3_pol_s_7x5"])
+    _check(f"{NAMES[form]} ff mode3 pol p", _np(lp), gold["ff_mode3_pol_p_7x5"])
+
+
+@pytest.mark.parametrize("kw", [
+    dict(flux_point=1, breakup_mode=4, use_polarized=0),
+    dict(flux_point=0, breakup_mode=1, use_polarized=1),
+    dict(Z=54, A=129, R=5.4889, a=0.54, sqrts=5440.0, flux_point=0, breakup_mode=3, use_polarized=0),
+    dict(Z=8, A=16, R=2.4809, a=0.54, sqrts=7000.0, flux_point=0, breakup_mode=2, use_polarized=1),
+    # Au-Au 200 GeV: the rows' nodes are dense around the P(min(b, 20)) clamp
+    # kink at b = 20 fm, which the cubic shared-grid interpolation smooths
+    # (stated bound 3e-5 at h = 0.2 fm)
+    dict(Z=79, A=197, R=6.3691, a=0.54, sqrts=200.0, flux_point=1, breakup_mode=2, use_polarized=1,
+         mmin=1.0, mmax=10.0, ymin=-3.0, ymax=3.0, tol=3e-5),
+    # asymmetric Y range: every point has its own k2 row
+    dict(flux_point=0, breakup_mode=2, use_polarized=0, ymin=-5.0, ymax=3.0),
+])
+def test_mfma_modes_vs_oracle(gpu_ctx, oracle_ctx, kw):
+    kw = dict(kw)
+    tol = kw.pop("tol", MFMA_TOL)
+    kw.setdefault("nm", 6)
+    kw.setdefault("ny", 5)
+    g = gpu_ctx(**kw)
+    o = oracle_ctx(**kw)
+    ref, nbad = o.lumi_grid()
+    assert nbad == 0
+    for form in FORMS:
+        out = g.lumi_grid(form)
+        if kw.get("use_polarized"):
+            _check(f"{NAMES[form]} {kw} s", _np(out[0]), ref[0], tol)
+            _check(f"{NAMES[form]} {kw} p", _np(out[1]), ref[1], tol)
+        else:
+            _check(f"{NAMES[form]} {kw}", _np(out), ref, tol)
+
+
+def test_mfma_config2_full_grid(gpu_ctx, oracle_ctx):
+    """1001 x 121 XNXN form-factor grid: every bin against the FP64 GPU form
+    (itself within 1e-9 of the oracle), and a sample of bins against the
+    oracle directly."""
+    g = gpu_ctx(flux_point=0, breakup_mode=2, nm=1001, ny=121, nz=100)
+    L64 = _np(g.lumi_grid(U.FORM_FP64))
+    o = oracle_ctx(flux_point=0, breakup_mode=2, nm=1001, ny=121, nz=100)
+    rng = np.random.default_rng(9)
+    dm, dy = (50 - 3.56) / 1001, 12.0 / 121
+    ims = np.concatenate([rng.integers(0, 1001, 12), [0, 1000, 1000, 500]])
+    iys = np.concatenate([rng.integers(0, 121, 12), [120, 120, 0, 60]])
+    ref = np.array([o.lumi(3.56 + dm * i, -6.0 + dy * j) * dm * dy for i, j in zip(ims, iys)])
+    for form in FORMS:
+        L = _np(g.lumi_grid(form))
+        assert np.all(np.isfinite(L)) and np.all(L > 0)
+        e = _check(f"{NAMES[form]} 1001x121 vs FP64 form", L, L64)
+        im, iy = np.unravel_index(np.argmax(e), e.shape)
+        print(f"  worst bin im={im} iy={iy}: {e[im, iy]:.2e}; ksteps {g.info.mfma_ksteps}")
+        _check(f"{NAMES[form]} 1001x121 sampled vs oracle", L[ims, iys], ref)
+
+
+def test_mfma_spacing_controls_error(gpu_ctx, oracle_ctx):
+    kw = dict(flux_point=0, breakup_mode=2, nm=9, ny=7)
+    g = gpu_ctx(**kw)
+    ref, _ = oracle_ctx(**kw).lumi_grid()
+    errs = []
+    try:
+        for h in (0.8, 0.4, 0.1):
+            g.set_mfma_spacing(h)
+            assert g.info.mfma_spacing == h
+            errs.append(relerr(_np(g.lumi_grid(U.FORM_MFMA_FP32)), ref).max())
+    finally:
+        g.set_mfma_spacing(0.2)
+    print("max rel err at h = 0.8, 0.4, 0.1 fm:", errs)
+    assert errs[2] < errs[1] < errs[0]
+    assert errs[2] < 2e-6
+    with pytest.raises(U.UpcError):
+        g.set_mfma_spacing(0.001)

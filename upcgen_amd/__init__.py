@@ -10,13 +10,13 @@ from __future__ import annotations
 import ctypes as C
 import os
 
-__all__ = ["Config", "Info", "Context", "load", "FORM_FP64", "FORM_MFMA", "FORM_SURROGATE",
+__all__ = ["Config", "Info", "Context", "load", "FORM_FP64", "FORM_MFMA", "FORM_SURROGATE", "FORM_MFMA_FP32",
            "UpcError", "LIB_PATH"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libupcgen_amd.so")
 
-FORM_FP64, FORM_MFMA, FORM_SURROGATE = 0, 1, 2
+FORM_FP64, FORM_MFMA, FORM_SURROGATE, FORM_MFMA_FP32 = 0, 1, 2, 3
 
 
 class UpcError(RuntimeError):
@@ -57,7 +57,7 @@ class Info(C.Structure):
     _fields_ = [("g1", C.c_double), ("g2", C.c_double), ("factor", C.c_double),
                 ("mNucl", C.c_double), ("rho0", C.c_double), ("csNN", C.c_double),
                 ("p20", C.c_double), ("qags_calls", C.c_longlong), ("qags_bad", C.c_longlong),
-                ("qags_evals", C.c_longlong)]
+                ("qags_evals", C.c_longlong), ("mfma_ksteps", C.c_longlong), ("mfma_spacing", C.c_double)]
 
 
 class EventOpts(C.Structure):
@@ -68,7 +68,7 @@ class EventOpts(C.Structure):
 class Timings(C.Structure):
     _fields_ = [("ms", C.c_double * 8), ("launches", C.c_longlong * 8)]
 
-PHASES = ("tables", "flux", "qags", "lumi", "xsec", "pdf")
+PHASES = ("tables", "flux", "qags", "lumi", "xsec", "pdf", "project")
 
 
 # Every symbol declared in include/upc_lumi.h with its ctypes signature.
@@ -82,6 +82,7 @@ SIGNATURES = {
     "upc_ctx_rebuild_tables": (_i, [_vp, _vp]),
     "upc_ctx_set_timing": (_i, [_vp, _i]),
     "upc_ctx_timings": (_i, [_vp, C.POINTER(Timings), _i]),
+    "upc_ctx_set_mfma_spacing": (_i, [_vp, _d]),
     "upc_flux_point": (_i, [_vp, _vp, _vp, _i, _vp, _vp]),
     "upc_flux_form": (_i, [_vp, _vp, _vp, _i, _vp, _vp, _vp]),
     "upc_form_factor": (_i, [_vp, _vp, _i, _i, _vp, _vp]),
@@ -175,6 +176,10 @@ class Context:
         inf = Info()
         _check(load().upc_ctx_info(self._ctx, C.byref(inf)), "upc_ctx_info")
         return inf
+
+    def set_mfma_spacing(self, h_fm):
+        """Node spacing [fm] of the MFMA forms' shared b grid."""
+        _check(load().upc_ctx_set_mfma_spacing(self._ctx, h_fm), "upc_ctx_set_mfma_spacing")
 
     def set_timing(self, enable=True):
         _check(load().upc_ctx_set_timing(self._ctx, int(enable)), "upc_ctx_set_timing")

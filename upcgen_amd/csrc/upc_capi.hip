@@ -38,7 +38,10 @@ PhaseScope::~PhaseScope()
 }
 
 // MFMA / surrogate forms live in their own translation units.
-int lumi_grid_mfma(upc_ctx* c, double* d_L, double* d_Ls, double* d_Lp, hipStream_t s);
+int lumi_grid_mfma(upc_ctx* c, int prec, double* d_L, double* d_Ls, double* d_Lp, hipStream_t s);
+int mfma_set_spacing(upc_ctx* c, double h);
+unsigned long long mfma_last_ksteps(upc_ctx* c);
+double mfma_spacing(upc_ctx* c);
 int lumi_grid_surrogate(upc_ctx* c, double* d_L, double* d_Ls, double* d_Lp, hipStream_t s);
 void free_mfma(upc_ctx* c);
 void free_surrogate(upc_ctx* c);
@@ -143,7 +146,15 @@ int upc_ctx_info(upc_ctx* c, upc_info* h)
   h->qags_calls = c->last_qags_calls;
   h->qags_bad = c->last_qags_bad;
   h->qags_evals = c->last_qags_evals;
+  h->mfma_ksteps = (long long)mfma_last_ksteps(c);
+  h->mfma_spacing = mfma_spacing(c);
   return 0;
+}
+
+int upc_ctx_set_mfma_spacing(upc_ctx* c, double h_fm)
+{
+  if (!c) return set_error("null context"), UPC_EINVAL;
+  return mfma_set_spacing(c, h_fm);
 }
 
 int upc_ctx_set_timing(upc_ctx* c, int enable)
@@ -239,7 +250,8 @@ int upc_lumi_grid(upc_ctx* c, int form, double* d_L, double* d_Ls, double* d_Lp,
       return lumi_points_fp64(c, nullptr, c->cfg.nm * c->cfg.ny, d_L, d_Ls, d_Lp, S(stream));
     }
     case UPC_FORM_MFMA:
-      return lumi_grid_mfma(c, d_L, d_Ls, d_Lp, S(stream));
+    case UPC_FORM_MFMA_FP32:
+      return lumi_grid_mfma(c, form, d_L, d_Ls, d_Lp, S(stream));
     case UPC_FORM_SURROGATE:
       return lumi_grid_surrogate(c, d_L, d_Ls, d_Lp, S(stream));
     default:

@@ -261,6 +261,16 @@ __device__ __forceinline__ double spline_eval(const double4* __restrict__ coef, 
   return c.x + d * (c.y + d * (c.z + d * c.w));
 }
 
+__device__ __forceinline__ double spline_deriv(const double4* __restrict__ coef, double x0, double dx,
+                                               double inv_dx, int n, double v)
+{
+  double xl;
+  const int i = spline_interval(x0, dx, inv_dx, n, v, &xl);
+  const double4 c = coef[i];
+  const double d = v - xl;
+  return c.y + d * (2.0 * c.z + 3.0 * d * c.w);
+}
+
 // ---------------------------------------------------------------------------
 // QUADPACK dqagse building blocks (QK21, dqpsrt, dqelg)
 // ---------------------------------------------------------------------------

@@ -14,6 +14,7 @@
 // QAGS tasks; the rest take the point-charge flux.
 #include "upc_internal.h"
 #include "upc_qags.h"
+#include "upc_flux.h"
 
 #include <climits>
 
@@ -22,20 +23,6 @@ namespace upc {
 constexpr int kQagsCap = 64;       // fast per-lane interval list
 constexpr int kQagsLimit = 1000;   // gsl_integration_qags limit of the reference
 constexpr int kQagsBigCap = 1000;
-
-struct FluxParams {
-  double bmin, R, g1, g2, factor, twoR;
-  int A, isPoint;
-  int mirror_ny;  // > 0: symmetric grid with ny Y bins (k2 rows shared with mirror bins)
-};
-
-// Row holding k2 of point p.
-__device__ __forceinline__ int k2_row(int p, int npts, int mirror_ny)
-{
-  if (mirror_ny <= 0) return npts + p;
-  const int im = p / mirror_ny, iy = p - im * mirror_ny;
-  return iy ? im * mirror_ny + (mirror_ny - iy) : npts + im;
-}
 
 __device__ __forceinline__ double flux_point(double b, double k, double g1, double factor)
 {
@@ -58,13 +45,6 @@ struct FluxFormIntegrand {
     return q * q * f / t * bessel_j1(b * q * (1.0 / kHc));
   }
 };
-
-__device__ __forceinline__ double row_node(double bmin, double logd, int j, double* bl, double* bh)
-{
-  *bl = bmin * exp(j * logd);
-  *bh = bmin * exp((j + 1.) * logd);
-  return (*bh + *bl) / 2.;
-}
 
 // ---------------------------------------------------------------------------
 // Grid points: lower bin edges (src/UpcCrossSection.cpp:540-550)
@@ -323,42 +303,6 @@ __global__ __launch_bounds__(64) void k_flux_qags_big(TS ts, const int* __restri
 // ---------------------------------------------------------------------------
 constexpr int kLumiWaves = 4;
 
-__device__ __forceinline__ double node_b2(double b1, double b2, double c)
-{
-  return fma(2.0 * b1 * b2, c, fma(b1, b1, b2 * b2));
-}
-
-template <bool POL>
-__device__ __forceinline__ void node_k(double b1, double b2, const double* cosk, const double* w5,
-                                       int mode, double p20, const double4* gaa, const double4* __restrict__ bk,
-                                       double* ks, double* kp)
-{
-  const double db = 20. / (kNB - 1);
-  double s = 0.0, p = 0.0;
-#pragma unroll
-  for (int k = 0; k < 5; ++k) {
-    const double c = cosk[k];
-    const double b = sqrt(node_b2(b1, b2, POL ? -c : c));
-    double breakup = 1., gaa_v = 1.;
-    if (b < 20.) {
-      if (mode != 1) breakup = spline_eval(bk, kBkMin, kDBk, 1.0 / kDBk, kNBkUsed, b);
-      gaa_v = spline_eval(gaa, 0.0, db, 1.0 / db, kNB, b);
-    } else if (mode != 1) {
-      breakup = p20;
-    }
-    if (POL) {
-      const double sn2 = 1.0 - c * c;  // sin^2 phi from the cosine: replaced below by the table
-      (void)sn2;
-      s += breakup * gaa_v * w5[k] * c * c;
-      p += breakup * gaa_v * w5[k] * w5[5 + k];
-    } else {
-      s += breakup * gaa_v * w5[k];
-    }
-  }
-  *ks = s;
-  *kp = p;
-}
-
 template <bool POL>
 __global__ __launch_bounds__(64 * kLumiWaves) void k_lumi_fp64(
     const double* __restrict__ pts, int npts, const double* __restrict__ row_logd,
@@ -485,21 +429,6 @@ __global__ __launch_bounds__(64 * kLumiWaves) void k_lumi_fp64(
 // Host side
 // ---------------------------------------------------------------------------
 static inline int nblk(long n, int bs) { return (int)((n + bs - 1) / bs); }
-
-static FluxParams flux_params(const upc_ctx* c, int mirror_ny = 0)
-{
-  FluxParams fp;
-  fp.mirror_ny = mirror_ny;
-  fp.R = c->cfg.R;
-  fp.bmin = c->cfg.flux_point ? c->cfg.R : 0.05 * c->cfg.R;
-  fp.g1 = c->g1;
-  fp.g2 = c->g2;
-  fp.factor = c->factor;
-  fp.twoR = 2. * c->cfg.R;
-  fp.A = c->cfg.A;
-  fp.isPoint = c->cfg.flux_point;
-  return fp;
-}
 
 static int qags_persistent_blocks()
 {
@@ -629,11 +558,8 @@ __global__ void k_copy_points(const double* __restrict__ M, const double* __rest
   pts[4 * i + 3] = 1.0;
 }
 
-// d_pts == nullptr: the grid points already in the work buffer.
-int lumi_points_fp64(upc_ctx* c, const double* d_pts, int npts, double* d_L, double* d_Ls, double* d_Lp,
-                     hipStream_t s)
+int prepare_flux(upc_ctx* c, const double* d_pts, int npts, FluxParams* fp, hipStream_t s)
 {
-  if (npts <= 0) return 0;
   int rc = ensure_work(c, (size_t)npts);
   if (rc) return rc;
   if (d_pts) {
@@ -643,7 +569,20 @@ int lumi_points_fp64(upc_ctx* c, const double* d_pts, int npts, double* d_L, dou
   const int mirror_ny = grid_mirror_ny(c, d_pts == nullptr);
   rc = run_flux(c, npts, mirror_ny, s);
   if (rc) return rc;
-  const FluxParams fp = flux_params(c, mirror_ny);
+  *fp = flux_params(c, mirror_ny);
+  return 0;
+}
+
+int read_flux_stats(upc_ctx* c, hipStream_t s) { return read_stats(c, s); }
+
+// d_pts == nullptr: the grid points already in the work buffer.
+int lumi_points_fp64(upc_ctx* c, const double* d_pts, int npts, double* d_L, double* d_Ls, double* d_Lp,
+                     hipStream_t s)
+{
+  if (npts <= 0) return 0;
+  FluxParams fp;
+  int rc = prepare_flux(c, d_pts, npts, &fp, s);
+  if (rc) return rc;
   const dim3 grid(nblk(npts, kLumiWaves)), block(64 * kLumiWaves);
   {
   PhaseScope ph(c, s, UPC_PHASE_LUMI);

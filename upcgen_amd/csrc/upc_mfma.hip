@@ -1,15 +1,682 @@
-// MFMA form of the luminosity grid (shared-grid F.K.F^T contraction).
-#include "upc_internal.h"
+// MFMA form of the two-photon luminosity grid.
+//
+// Reference: calcTwoPhotonLumi / calcTwoPhotonLumiPol,
+// src/UpcCrossSection.cpp:221-335 (the same sum the FP64 form evaluates node
+// by node).
+//
+// The flux rows are the FP64 rows of the FP64 form (point-charge flux and the
+// QAGS form-factor flux).  The reference's QAGS runs at epsabs = epsrel =
+// 1e-4, and where the flux is small its result is mostly quadrature error,
+// which dominates L in the large-k corner bins; so the rows themselves are
+// never approximated.  What is recast is the b1/b2/phi kernel: with
+//   W_i   = N(b_i, k) b_i (b_h - b_l)                (one flux row)
+//   g(b)  = P(min(b, 20)) G(b),  G = 1 for b >= 20
+//   K(b1, b2) = sum_k w_k g(|b1 + b2 e^{i phi_k}|),   D = K - S0
+// the node sum is L = 2 pi^2 M [S0 (sum W1)(sum W2) + sum_ij W1_i W2_j D_ij]
+// and D is interpolated on one uniform grid B_u = u h shared by every row:
+//   What[r][u] = sum_i W_i l_u(b_i)     (4-point Lagrange weights)
+//   L = 2 pi^2 M [S0 (sum W1)(sum W2) + What1^T D What2],  D_uv = D(B_u, B_v).
+// At b = 20 fm g is not smooth: P is clamped there (slope jump) and G_AA
+// switches from its spline to exactly 1 (value jump J = g(20) - g(20-)).
+// Interpolating that edge costs ~1e-5 of L at h = 0.2 fm, so it is split off:
+//   g = g_c + e,   e(b) = [b < 20] (-J + kappa(b)),
+//   kappa(b) = -g'(20-) (20 - b) (1 - x^2)^2,  x = (20 - b) / delta <= 1,
+// which leaves g_c continuous with a continuous slope at 20 fm.  Only D_c
+// (from g_c) is interpolated; the edge part
+//   sum_ij W1_i W2_j sum_k w_k e(b_k(i, j))
+// is evaluated exactly at the rows' own node pairs (kernel k_mfma_edge): for
+// fixed (i, k) the j with b_k < 20 form one contiguous range (b_k^2 is a
+// parabola in b2), so the J term is a difference of prefix sums of W2 and
+// kappa is summed over the few nodes of that range with b_k > 20 - delta.
+// D_c is symmetric, vanishes for |B_u - B_v| >= 20 fm (every node is then at or
+// beyond 20 fm, where K = S0 in the same arithmetic) and for B >= 20 /
+// sqrt(1 - c^2) with c the extreme node cosine, whatever the partner.  So the
+// contraction is a banded GEMM  Y = What2 D  over all points at once, fused
+// with the dot  Y . What1.  It runs on MFMA as split-bf16 (x = hi + lo,
+// hi*hi + hi*lo + lo*hi, FP32 accumulation) or as FP32 MFMA; the dot and the
+// S0 term are FP64.  Each What row is scaled by its max |value| (kept in FP64)
+// so the FP32 range never limits a row.
+#include "upc_flux.h"
+
+#include <cmath>
+#include <cstdint>
 
 namespace upc {
 
-int lumi_grid_mfma(upc_ctx* c, double* d_L, double* d_Ls, double* d_Lp, hipStream_t s)
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x8 __attribute__((ext_vector_type(8)));
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+
+constexpr int kMfmaWaves = 4;      // waves per block in the contraction kernel
+constexpr double kEdgeWidth = 2.0;  // delta: support of the slope-matching ramp below 20 fm
+
+// e(b) of the edge split; J = edge[0], -g'(20-) = edge[1].
+__device__ __forceinline__ double edge_e(double b, double J, double s)
 {
-  (void)c; (void)d_L; (void)d_Ls; (void)d_Lp; (void)s;
-  set_error("MFMA form not available in this build");
-  return UPC_ESTATE;
+  if (b >= 20.) return 0.;
+  double e = -J;
+  const double x = (20. - b) * (1.0 / kEdgeWidth);
+  if (x < 1.) {
+    const double t = 1. - x * x;
+    e += s * (20. - b) * t * t;
+  }
+  return e;
 }
 
-void free_mfma(upc_ctx* c) { (void)c; }
+// J and -g'(20-) of g(b) = P(b) G(b) (P = 1 in mode 1) from the splines.
+__global__ void k_mfma_edge_params(int mode, const double4* __restrict__ gaa, const double4* __restrict__ bk,
+                                   const DevScalars* __restrict__ sc, double* __restrict__ edge)
+{
+  const double db = 20. / (kNB - 1);
+  const double G = spline_eval(gaa, 0.0, db, 1.0 / db, kNB, 20.0);
+  const double dG = spline_deriv(gaa, 0.0, db, 1.0 / db, kNB, 20.0);
+  double P = 1., dP = 0.;
+  if (mode != 1) {
+    P = sc->p20;
+    dP = spline_deriv(bk, kBkMin, kDBk, 1.0 / kDBk, kNBkUsed, 20.0);
+  }
+  edge[0] = P - P * G;  // g(20) = P(20) exactly; g(20-) = P(20) G(20)
+  edge[1] = -(dP * G + P * dG);
+}
+
+struct MfmaGrid {
+  double h;       // node spacing of the shared grid (fm)
+  double inv_h;
+  double bstar;   // D(b, .) = 0 for b >= bstar
+  int nu;         // nodes in use
+  int nu_pad;     // row length (multiple of 32)
+  int band;       // D_uv = 0 for |u - v| >= band
+  int npol;       // 1, or 2 for the polarised pair (D_s, D_p)
+};
+
+struct MfmaState {
+  double h = 0.2;
+  int prec = UPC_FORM_MFMA;
+  MfmaGrid g{};
+  size_t d_cap = 0, rows_cap = 0;
+  uint16_t* d_Dh = nullptr;   // [npol][nu_pad][nu_pad] bf16 hi
+  uint16_t* d_Dl = nullptr;   // bf16 lo
+  float* d_D32 = nullptr;     // FP32 variant
+  uint16_t* d_Wh = nullptr;   // [rows][nu_pad]
+  uint16_t* d_Wl = nullptr;
+  float* d_W32 = nullptr;
+  double* d_wscale = nullptr; // [rows]
+  double* d_wsum = nullptr;   // [rows] sum of all 120 W_i
+  int* d_ulast = nullptr;     // [rows] last nonzero What column
+  double* d_edge_sum = nullptr;  // [npol][points] exact edge part
+  double* d_edge = nullptr;      // J, -g'(20-)
+  unsigned long long* d_ksteps = nullptr;
+  unsigned long long last_ksteps = 0;
+};
+
+static MfmaState* state(upc_ctx* c)
+{
+  if (!c->mfma) c->mfma = new MfmaState();
+  return static_cast<MfmaState*>(c->mfma);
+}
+
+__device__ __forceinline__ uint16_t to_bf16(float f)
+{
+  const uint32_t u = __float_as_uint(f);
+  return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);  // round to nearest even (finite input)
+}
+__device__ __forceinline__ float from_bf16(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+
+__device__ __forceinline__ void split_store(double x, uint16_t* hi, uint16_t* lo, float* f32, long i)
+{
+  if (f32) {
+    f32[i] = (float)x;
+  } else {
+    const uint16_t h = to_bf16((float)x);
+    hi[i] = h;
+    lo[i] = to_bf16((float)(x - (double)from_bf16(h)));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// D on the shared grid
+// ---------------------------------------------------------------------------
+template <bool POL>
+__global__ __launch_bounds__(256) void k_mfma_kernel_matrix(MfmaGrid g, int mode, const double4* __restrict__ gaa,
+                                                             const double4* __restrict__ bk,
+                                                             const DevScalars* __restrict__ sc,
+                                                             const double* __restrict__ edge,
+                                                             uint16_t* __restrict__ Dh, uint16_t* __restrict__ Dl,
+                                                             float* __restrict__ D32)
+{
+  const long n = (long)g.nu_pad * g.nu_pad;
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n) return;
+  const int u = (int)(idx / g.nu_pad), v = (int)(idx - (long)u * g.nu_pad);
+  double cosk[5], w5[10];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    const double c = cos(kPi * kGLx5[k]), sn = sin(kPi * kGLx5[k]);
+    cosk[k] = c;
+    w5[k] = kGLw5[k];
+    w5[5 + k] = sn * sn;
+  }
+  const double p20 = sc->p20;
+  double ds = 0, dp = 0;
+  if (u < g.nu && v < g.nu && abs(u - v) < g.band) {
+    const double br = mode != 1 ? p20 : 1.;
+    double s0s = 0, s0p = 0;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      if (POL) {
+        s0s += br * 1. * w5[k] * cosk[k] * cosk[k];
+        s0p += br * 1. * w5[k] * w5[5 + k];
+      } else {
+        s0s += br * 1. * w5[k];
+      }
+    }
+    const double db = 20. / (kNB - 1);
+    const double J = edge[0], sl = edge[1];
+    const double b1 = u * g.h, b2 = v * g.h;
+    double ks = 0, kp = 0;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      const double c = cosk[k];
+      const double b = sqrt(node_b2(b1, b2, POL ? -c : c));
+      double gc = br;
+      if (b < 20.) {
+        const double breakup = mode != 1 ? spline_eval(bk, kBkMin, kDBk, 1.0 / kDBk, kNBkUsed, b) : 1.;
+        gc = breakup * spline_eval(gaa, 0.0, db, 1.0 / db, kNB, b) - edge_e(b, J, sl);
+      }
+      if (POL) {
+        ks += gc * w5[k] * c * c;
+        kp += gc * w5[k] * w5[5 + k];
+      } else {
+        ks += gc * w5[k];
+      }
+    }
+    ds = ks - s0s;
+    dp = kp - s0p;
+  }
+  split_store(ds, Dh, Dl, D32, idx);
+  if (POL) split_store(dp, Dh ? Dh + n : nullptr, Dl ? Dl + n : nullptr, D32 ? D32 + n : nullptr, idx);
+}
+
+// ---------------------------------------------------------------------------
+// Projection of the flux rows onto the shared grid: one wave per row.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_mfma_project(int nrows, const double* __restrict__ row_logd,
+                                                      const double* __restrict__ flux, FluxParams fp, MfmaGrid g,
+                                                      uint16_t* __restrict__ Wh, uint16_t* __restrict__ Wl,
+                                                      float* __restrict__ W32, double* __restrict__ wscale,
+                                                      double* __restrict__ wsum, int* __restrict__ ulast)
+{
+  __shared__ double sW[4][kNB1], sL[4][kNB1][4];
+  __shared__ int sI[4][kNB1];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int r0 = blockIdx.x * 4 + wv;
+  const bool live = r0 < nrows;
+  const int r = live ? r0 : nrows - 1;
+  const double logd = row_logd[r];
+  double tot = 0;
+  for (int j = lane; j < kNB1; j += 64) {
+    double bl, bh;
+    const double b = row_node(fp.bmin, logd, j, &bl, &bh);
+    const double w = flux[(long)r * kNB1 + j] * b * (bh - bl);
+    tot += w;
+    sW[wv][j] = w;
+    int i0 = INT_MAX;
+    if (b < g.bstar) {
+      const double t = b * g.inv_h;
+      i0 = (int)floor(t) - 1;
+      i0 = i0 < 0 ? 0 : (i0 > g.nu - 4 ? g.nu - 4 : i0);
+      const double x = t - i0;  // nodes at x = 0, 1, 2, 3
+      sL[wv][j][0] = -(x - 1.) * (x - 2.) * (x - 3.) / 6.;
+      sL[wv][j][1] = x * (x - 2.) * (x - 3.) / 2.;
+      sL[wv][j][2] = -x * (x - 1.) * (x - 3.) / 2.;
+      sL[wv][j][3] = x * (x - 1.) * (x - 2.) / 6.;
+    }
+    sI[wv][j] = i0;
+  }
+  for (int o = 32; o > 0; o >>= 1) tot += __shfl_xor(tot, o);
+  __syncthreads();
+  // gather: node i feeds columns i0_i .. i0_i + 3; i0 is nondecreasing in i.
+  // Two passes over the columns: the row maximum first, then the scaled store.
+  auto column = [&](int u) {
+    int lo = 0, hi = kNB1;  // first i with i0_i >= u - 3
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (sI[wv][mid] < u - 3) lo = mid + 1;
+      else hi = mid;
+    }
+    double a = 0;
+    for (int i = lo; i < kNB1 && sI[wv][i] <= u; ++i) a += sW[wv][i] * sL[wv][i][u - sI[wv][i]];
+    return a;
+  };
+  double amax = 0;
+  int last = -1;
+  for (int u = lane; u < g.nu; u += 64) {
+    const double a = column(u);
+    amax = fmax(amax, fabs(a));
+    if (a != 0) last = u;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    amax = fmax(amax, __shfl_xor(amax, o));
+    last = max(last, __shfl_xor(last, o));
+  }
+  if (!live) return;
+  const double inv = amax > 0 ? 1.0 / amax : 0.0;
+  for (int u = lane; u < g.nu_pad; u += 64)
+    split_store(u < g.nu ? column(u) * inv : 0.0, Wh, Wl, W32, (long)r * g.nu_pad + u);
+  if (lane == 0) {
+    wscale[r] = amax;
+    wsum[r] = tot;
+    ulast[r] = last;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Edge part: sum_i W1_i sum_k w_k t_k sum_{j: b_k(i,j) < 20} W2_j e(b_k(i,j)),
+// t_k = 1 (unpolarised) or cos^2 / sin^2 (polarised), at the rows' own nodes.
+// One wave per point; lanes take (i, k) pairs.
+// ---------------------------------------------------------------------------
+template <bool POL>
+__global__ __launch_bounds__(256) void k_mfma_edge(int npts, const double* __restrict__ row_logd,
+                                                   const double* __restrict__ flux, FluxParams fp,
+                                                   const double* __restrict__ edge, double* __restrict__ out)
+{
+  __shared__ double sb1[4][kNB1], sb2[4][kNB2], sw1[4][kNB1], sw2[4][kNB2], spre[4][kNB2 + 1];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int p0 = blockIdx.x * 4 + wv;
+  const bool live = p0 < npts;
+  const int p = live ? p0 : npts - 1;
+  const int r2 = k2_row(p, npts, fp.mirror_ny);
+  const double logd1 = row_logd[p], logd2 = row_logd[r2];
+  for (int j = lane; j < kNB1; j += 64) {
+    double bl, bh;
+    const double b1 = row_node(fp.bmin, logd1, j, &bl, &bh);
+    sb1[wv][j] = b1;
+    sw1[wv][j] = flux[(long)p * kNB1 + j] * b1 * (bh - bl);
+    const double b2 = row_node(fp.bmin, logd2, j, &bl, &bh);
+    sb2[wv][j] = b2;
+    sw2[wv][j] = flux[(long)r2 * kNB1 + j] * b2 * (bh - bl);
+  }
+  __syncthreads();
+  if (lane == 0) {
+    spre[wv][0] = 0;
+    for (int j = 0; j < kNB2; ++j) spre[wv][j + 1] = spre[wv][j] + sw2[wv][j];
+  }
+  __syncthreads();
+  if (!live) return;
+  const double J = edge[0], sl = edge[1];
+  // e = -J exactly below b = 20 - delta: the ring split may use b^2 directly
+  constexpr double kCore2 = (20. - kEdgeWidth) * (20. - kEdgeWidth);
+  double cosk[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) cosk[k] = cos(kPi * kGLx5[k]);
+  double acc_s = 0, acc_p = 0;
+  for (int t = lane; t < kNB1 * 5; t += 64) {
+    const int i = t / 5, k = t - 5 * (t / 5);
+    double cs = cosk[0];
+#pragma unroll
+    for (int q = 1; q < 5; ++q) cs = k == q ? cosk[q] : cs;
+    const double c = POL ? -cs : cs;
+    const double b1 = sb1[wv][i];
+    auto bk2 = [&](int j) { return node_b2(b1, sb2[wv][j], c); };
+    // sqrt(x) < 20 exactly as the node loop tests it: x >= 400 is never
+    // inside, and only x within rounding of 400 needs the square root
+    auto inside = [&](double x) { return x < 400. && (x < 399.99999 || sqrt(x) < 20.); };
+    // b_k^2 is smallest at b2 = -b1 c: the last node below it or the first above
+    const double vtx = -b1 * c;
+    int lo = 0, hi = kNB2;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (sb2[wv][mid] < vtx) lo = mid + 1;
+      else hi = mid;
+    }
+    int jc = -1;
+    const double xlo = lo < kNB2 ? bk2(lo) : 1e300, xlm = lo > 0 ? bk2(lo - 1) : 1e300;
+    if (inside(xlo)) jc = lo;
+    if (inside(xlm) && (jc < 0 || xlm < xlo)) jc = lo - 1;
+    if (jc < 0) continue;
+    // first node in [0, jc] that passes (b_k falls towards jc)
+    auto first_in = [&](auto pass) {
+      int a = 0, b = jc;
+      while (a < b) {
+        const int mid = (a + b) >> 1;
+        if (pass(bk2(mid))) b = mid;
+        else a = mid + 1;
+      }
+      return a;
+    };
+    // last node in [jc, n) that passes (b_k rises after jc)
+    auto last_in = [&](auto pass) {
+      int a = jc, b = kNB2 - 1;
+      while (a < b) {
+        const int mid = (a + b + 1) >> 1;
+        if (pass(bk2(mid))) a = mid;
+        else b = mid - 1;
+      }
+      return a;
+    };
+    auto core = [&](double x) { return x < kCore2; };
+    const int jlo = first_in(inside), jhi = last_in(inside);
+    double sum = -J * (spre[wv][jhi + 1] - spre[wv][jlo]);
+    const int r0 = jlo, r1 = jhi + 1;  // ring: [r0, l1) and [l2, r1), where 20 - delta <= b_k < 20
+    int l1 = r1, l2 = r1;
+    if (core(bk2(jc))) {
+      l1 = first_in(core);
+      l2 = last_in(core) + 1;
+    }
+    for (int j = r0; j < l1; ++j) sum += sw2[wv][j] * (edge_e(sqrt(bk2(j)), J, sl) + J);
+    for (int j = l2; j < r1; ++j) sum += sw2[wv][j] * (edge_e(sqrt(bk2(j)), J, sl) + J);
+    const double w = sw1[wv][i] * kGLw5[k] * sum;
+    if (POL) {
+      acc_s += w * cs * cs;
+      acc_p += w * (1.0 - cs * cs);
+    } else {
+      acc_s += w;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    acc_s += __shfl_xor(acc_s, o);
+    if (POL) acc_p += __shfl_xor(acc_p, o);
+  }
+  if (lane == 0) {
+    out[p] = acc_s;
+    if (POL) out[npts + p] = acc_p;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Banded GEMM Y = What2 D on MFMA, fused with the dot Y . What1.
+//
+// A wave owns 16 points.  16x16x32 bf16 MFMA (or 16x16x4 f32) operand
+// layout: lane l supplies row/column (l & 15) and the 8 consecutive k values
+// 8 (l >> 4) .. 8 (l >> 4) + 7 of a 32-wide k step (A and B use the same k
+// assignment, so the sum over k is the full product); the accumulator holds
+// C[4 (l >> 4) + i][l & 15], i = 0..3.
+// ---------------------------------------------------------------------------
+template <int PREC, bool POL>
+__global__ __launch_bounds__(64 * kMfmaWaves) void k_mfma_lumi(
+    const double* __restrict__ pts, int npts, FluxParams fp, MfmaGrid g, const uint16_t* __restrict__ Dh,
+    const uint16_t* __restrict__ Dl, const float* __restrict__ D32, const uint16_t* __restrict__ Wh,
+    const uint16_t* __restrict__ Wl, const float* __restrict__ W32, const double* __restrict__ wscale,
+    const double* __restrict__ wsum, const int* __restrict__ ulast, int mode, const DevScalars* __restrict__ sc,
+    const double* __restrict__ edge_sum, double* __restrict__ L, double* __restrict__ Ls, double* __restrict__ Lp,
+    unsigned long long* __restrict__ ksteps)
+{
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int p0 = (blockIdx.x * kMfmaWaves + wv) * 16;
+  if (p0 >= npts) return;
+  const int m = lane & 15, g4 = lane >> 4;
+  double s0s = 0, s0p = 0;  // the phi sum beyond 20 fm, as in the FP64 form
+  {
+    const double br = mode != 1 ? sc->p20 : 1.;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      if (POL) {
+        const double c = cos(kPi * kGLx5[k]), sn = sin(kPi * kGLx5[k]);
+        s0s += br * 1. * kGLw5[k] * c * c;
+        s0p += br * 1. * kGLw5[k] * (sn * sn);
+      } else {
+        s0s += br * 1. * kGLw5[k];
+      }
+    }
+  }
+  const int pa = min(p0 + m, npts - 1);
+  const long rowA = (long)k2_row(pa, npts, fp.mirror_ny) * g.nu_pad;
+  // extents of the 16 rows: A (k2 rows) bounds k, the k1 rows bound u
+  int kend = ulast[k2_row(pa, npts, fp.mirror_ny)] + 1;
+  int uend = ulast[pa] + 1;
+  for (int o = 8; o > 0; o >>= 1) {
+    kend = max(kend, __shfl_xor(kend, o));
+    uend = max(uend, __shfl_xor(uend, o));
+  }
+  kend = max(kend, __shfl_xor(kend, 16));
+  kend = max(kend, __shfl_xor(kend, 32));
+  uend = max(uend, __shfl_xor(uend, 16));
+  uend = max(uend, __shfl_xor(uend, 32));
+  const long nn = (long)g.nu_pad * g.nu_pad;
+  double dot_s[4] = {0, 0, 0, 0}, dot_p[4] = {0, 0, 0, 0};
+  long rowC[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) rowC[i] = (long)min(p0 + 4 * g4 + i, npts - 1) * g.nu_pad;
+  unsigned long long nk = 0;
+  const int ntiles = (uend + 15) >> 4;
+  const int kbmax = (kend + 31) >> 5;
+  for (int t = 0; t < ntiles; ++t) {
+    const int u0 = t * 16;
+    const int kb0 = max(0, (u0 - g.band + 1) >> 5);
+    const int kb1 = min(kbmax, (u0 + 15 + g.band - 1) / 32 + 1);
+    f32x4 acc = {0, 0, 0, 0}, accp = {0, 0, 0, 0};
+    const long rowB = (long)(u0 + m) * g.nu_pad;
+    for (int kb = kb0; kb < kb1; ++kb) {
+      const int k = kb * 32 + g4 * 8;
+      if (PREC == UPC_FORM_MFMA) {
+        const bf16x8 ah = __builtin_bit_cast(bf16x8, *(const u16x8*)(Wh + rowA + k));
+        const bf16x8 al = __builtin_bit_cast(bf16x8, *(const u16x8*)(Wl + rowA + k));
+        const bf16x8 bh = __builtin_bit_cast(bf16x8, *(const u16x8*)(Dh + rowB + k));
+        const bf16x8 bl = __builtin_bit_cast(bf16x8, *(const u16x8*)(Dl + rowB + k));
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc, 0, 0, 0);
+        if (POL) {
+          const bf16x8 ph = __builtin_bit_cast(bf16x8, *(const u16x8*)(Dh + nn + rowB + k));
+          const bf16x8 pl = __builtin_bit_cast(bf16x8, *(const u16x8*)(Dl + nn + rowB + k));
+          accp = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, ph, accp, 0, 0, 0);
+          accp = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, pl, accp, 0, 0, 0);
+          accp = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, ph, accp, 0, 0, 0);
+        }
+      } else {
+        const f32x8 a = *(const f32x8*)(W32 + rowA + k);
+        const f32x8 b = *(const f32x8*)(D32 + rowB + k);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[e], b[e], acc, 0, 0, 0);
+        if (POL) {
+          const f32x8 bp = *(const f32x8*)(D32 + nn + rowB + k);
+#pragma unroll
+          for (int e = 0; e < 8; ++e) accp = __builtin_amdgcn_mfma_f32_16x16x4f32(a[e], bp[e], accp, 0, 0, 0);
+        }
+      }
+    }
+    nk += kb1 > kb0 ? kb1 - kb0 : 0;
+    const int u = u0 + m;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      float w1;
+      if (PREC == UPC_FORM_MFMA) w1 = from_bf16(Wh[rowC[i] + u]) + from_bf16(Wl[rowC[i] + u]);
+      else w1 = W32[rowC[i] + u];
+      dot_s[i] += (double)acc[i] * (double)w1;
+      if (POL) dot_p[i] += (double)accp[i] * (double)w1;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    for (int o = 8; o > 0; o >>= 1) {
+      dot_s[i] += __shfl_xor(dot_s[i], o);
+      if (POL) dot_p[i] += __shfl_xor(dot_p[i], o);
+    }
+  }
+  if (lane == 0) atomicAdd(ksteps, nk);
+  if (m != 0) return;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int q = p0 + 4 * g4 + i;
+    if (q >= npts) continue;
+    const int r2 = k2_row(q, npts, fp.mirror_ny);
+    const double pre = 2 * kPi * kPi * pts[4 * q] * pts[4 * q + 2] * pts[4 * q + 3];
+    const double ww = wsum[q] * wsum[r2], scale = wscale[q] * wscale[r2];
+    if (POL) {
+      Ls[q] = pre * (s0s * ww + scale * dot_s[i] + edge_sum[q]);
+      Lp[q] = pre * (s0p * ww + scale * dot_p[i] + edge_sum[npts + q]);
+    } else {
+      L[q] = pre * (s0s * ww + scale * dot_s[i] + edge_sum[q]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+static inline int nblk(long n, int bs) { return (int)((n + bs - 1) / bs); }
+
+static MfmaGrid make_grid(double h, bool pol)
+{
+  MfmaGrid g{};
+  g.h = h;
+  g.inv_h = 1.0 / h;
+  // extreme node cosine: the smallest b of a pair is at cos = -1 side
+  const double cext = pol ? std::cos(kPi * 0.1488743389816312) : -std::cos(kPi * 0.9739065285171717);
+  g.bstar = 20.0 / std::sqrt(1.0 - cext * cext) * (1.0 + 1e-12);
+  g.nu = (int)std::floor(g.bstar / h) + 4;
+  g.nu_pad = (g.nu + 31) / 32 * 32;
+  g.band = (int)std::ceil(20.0 / h) + 1;
+  g.npol = pol ? 2 : 1;
+  return g;
+}
+
+static void free_buffers(MfmaState* st, bool rows_only)
+{
+  void* rows[] = {st->d_Wh, st->d_Wl, st->d_W32, st->d_wscale, st->d_wsum, st->d_ulast, st->d_edge_sum};
+  for (void* p : rows)
+    if (p) (void)hipFree(p);
+  st->d_Wh = st->d_Wl = nullptr;
+  st->d_W32 = nullptr;
+  st->d_wscale = st->d_wsum = nullptr;
+  st->d_ulast = nullptr;
+  st->d_edge_sum = nullptr;
+  st->rows_cap = 0;
+  if (rows_only) return;
+  void* dm[] = {st->d_Dh, st->d_Dl, st->d_D32, st->d_ksteps, st->d_edge};
+  for (void* p : dm)
+    if (p) (void)hipFree(p);
+  st->d_Dh = st->d_Dl = nullptr;
+  st->d_D32 = nullptr;
+  st->d_ksteps = nullptr;
+  st->d_edge = nullptr;
+  st->d_cap = 0;
+}
+
+static int ensure_buffers(MfmaState* st, size_t nrows)
+{
+  const MfmaGrid& g = st->g;
+  const size_t dn = (size_t)g.npol * g.nu_pad * g.nu_pad;
+  if (dn > st->d_cap) {
+    free_buffers(st, false);
+    UPC_HIP(hipMalloc(&st->d_Dh, dn * 2));
+    UPC_HIP(hipMalloc(&st->d_Dl, dn * 2));
+    UPC_HIP(hipMalloc(&st->d_D32, dn * 4));
+    UPC_HIP(hipMalloc(&st->d_ksteps, sizeof(unsigned long long)));
+    UPC_HIP(hipMalloc(&st->d_edge, 2 * sizeof(double)));
+    st->d_cap = dn;
+  }
+  const size_t wn = nrows * g.nu_pad;
+  if (wn > st->rows_cap) {
+    free_buffers(st, true);
+    if (st->prec == UPC_FORM_MFMA) {
+      UPC_HIP(hipMalloc(&st->d_Wh, wn * 2));
+      UPC_HIP(hipMalloc(&st->d_Wl, wn * 2));
+    } else {
+      UPC_HIP(hipMalloc(&st->d_W32, wn * 4));
+    }
+    UPC_HIP(hipMalloc(&st->d_wscale, nrows * sizeof(double)));
+    UPC_HIP(hipMalloc(&st->d_wsum, nrows * sizeof(double)));
+    UPC_HIP(hipMalloc(&st->d_ulast, nrows * sizeof(int)));
+    UPC_HIP(hipMalloc(&st->d_edge_sum, 2 * nrows * sizeof(double)));
+    st->rows_cap = wn;
+  }
+  return 0;
+}
+
+int mfma_set_spacing(upc_ctx* c, double h)
+{
+  if (!(h >= 0.05 && h <= 2.0)) {
+    set_error("MFMA grid spacing must be in [0.05, 2] fm");
+    return UPC_EINVAL;
+  }
+  MfmaState* st = state(c);
+  st->h = h;
+  return 0;
+}
+
+unsigned long long mfma_last_ksteps(upc_ctx* c) { return c->mfma ? state(c)->last_ksteps : 0; }
+double mfma_spacing(upc_ctx* c) { return state(c)->h; }
+
+int lumi_points_mfma(upc_ctx* c, int prec, const double* d_pts, int npts, double* d_L, double* d_Ls,
+                     double* d_Lp, hipStream_t s)
+{
+  if (npts <= 0) return 0;
+  MfmaState* st = state(c);
+  const bool pol = c->cfg.use_polarized != 0;
+  const MfmaGrid g = make_grid(st->h, pol);
+  if (prec != st->prec || g.nu_pad != st->g.nu_pad || g.npol != st->g.npol) free_buffers(st, false);
+  st->prec = prec;
+  st->g = g;
+  FluxParams fp;
+  int rc = prepare_flux(c, d_pts, npts, &fp, s);
+  if (rc) return rc;
+  const int nrows = fp.mirror_ny > 0 ? npts + npts / fp.mirror_ny : 2 * npts;
+  rc = ensure_buffers(st, (size_t)nrows);
+  if (rc) return rc;
+  const bool split = prec == UPC_FORM_MFMA;
+  {
+    PhaseScope ph(c, s, UPC_PHASE_PROJECT);
+    const long dn = (long)g.nu_pad * g.nu_pad;
+    uint16_t* Dh = split ? st->d_Dh : nullptr;
+    uint16_t* Dl = split ? st->d_Dl : nullptr;
+    float* D32 = split ? nullptr : st->d_D32;
+    hipLaunchKernelGGL(k_mfma_edge_params, dim3(1), dim3(1), 0, s, c->cfg.breakup_mode, c->d_gaa, c->d_bk,
+                       c->d_scalars, st->d_edge);
+    if (pol)
+      hipLaunchKernelGGL(k_mfma_kernel_matrix<true>, dim3(nblk(dn, 256)), dim3(256), 0, s, g,
+                         c->cfg.breakup_mode, c->d_gaa, c->d_bk, c->d_scalars, st->d_edge, Dh, Dl, D32);
+    else
+      hipLaunchKernelGGL(k_mfma_kernel_matrix<false>, dim3(nblk(dn, 256)), dim3(256), 0, s, g,
+                         c->cfg.breakup_mode, c->d_gaa, c->d_bk, c->d_scalars, st->d_edge, Dh, Dl, D32);
+    hipLaunchKernelGGL(k_mfma_project, dim3(nblk(nrows, 4)), dim3(256), 0, s, nrows, c->work.d_row_logd,
+                       c->work.d_flux, fp, g, split ? st->d_Wh : nullptr, split ? st->d_Wl : nullptr,
+                       split ? nullptr : st->d_W32, st->d_wscale, st->d_wsum, st->d_ulast);
+    if (pol)
+      hipLaunchKernelGGL(k_mfma_edge<true>, dim3(nblk(npts, 4)), dim3(256), 0, s, npts, c->work.d_row_logd,
+                         c->work.d_flux, fp, st->d_edge, st->d_edge_sum);
+    else
+      hipLaunchKernelGGL(k_mfma_edge<false>, dim3(nblk(npts, 4)), dim3(256), 0, s, npts, c->work.d_row_logd,
+                         c->work.d_flux, fp, st->d_edge, st->d_edge_sum);
+    UPC_HIP(hipMemsetAsync(st->d_ksteps, 0, sizeof(unsigned long long), s));
+  }
+  {
+    PhaseScope ph(c, s, UPC_PHASE_LUMI);
+    const dim3 grid(nblk(npts, 16 * kMfmaWaves)), block(64 * kMfmaWaves);
+#define UPC_MFMA_LAUNCH(P, POLV)                                                                         \
+  hipLaunchKernelGGL((k_mfma_lumi<P, POLV>), grid, block, 0, s, c->work.d_points, npts, fp, g, st->d_Dh, \
+                     st->d_Dl, st->d_D32, st->d_Wh, st->d_Wl, st->d_W32, st->d_wscale, st->d_wsum,      \
+                     st->d_ulast, c->cfg.breakup_mode, c->d_scalars, st->d_edge_sum, d_L, d_Ls, d_Lp,       \
+                     st->d_ksteps)
+    if (split) {
+      if (pol) UPC_MFMA_LAUNCH(UPC_FORM_MFMA, true);
+      else UPC_MFMA_LAUNCH(UPC_FORM_MFMA, false);
+    } else {
+      if (pol) UPC_MFMA_LAUNCH(UPC_FORM_MFMA_FP32, true);
+      else UPC_MFMA_LAUNCH(UPC_FORM_MFMA_FP32, false);
+    }
+#undef UPC_MFMA_LAUNCH
+  }
+  UPC_HIP(hipGetLastError());
+  UPC_HIP(hipMemcpyAsync(&st->last_ksteps, st->d_ksteps, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+  return read_flux_stats(c, s);
+}
+
+int lumi_grid_mfma(upc_ctx* c, int prec, double* d_L, double* d_Ls, double* d_Lp, hipStream_t s)
+{
+  int rc = make_grid_points(c, s);
+  if (rc) return rc;
+  return lumi_points_mfma(c, prec, nullptr, c->cfg.nm * c->cfg.ny, d_L, d_Ls, d_Lp, s);
+}
+
+void free_mfma(upc_ctx* c)
+{
+  if (!c->mfma) return;
+  MfmaState* st = state(c);
+  free_buffers(st, false);
+  delete st;
+  c->mfma = nullptr;
+}
 
 }  // namespace upc
