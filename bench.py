@@ -32,6 +32,14 @@ BF16_PEAK_TFLOPS = 2500.0
 FP32_MFMA_PEAK_TFLOPS = 157.3
 # one MFMA tile step of the MFMA forms: 16 points x 16 columns x 32 deep
 KSTEP_FLOP = 2 * 16 * 16 * 32
+# FP64 operations of one form-factor flux integrand evaluation q^2 F(t)/t J1(bq/hc)
+# (fluxFormIntegrand, src/UpcCrossSection.cpp:181-191) plus its QK21 accumulation,
+# counted on the J1 branch x > 8 that most evaluations take (FMA = 2):
+#   t and the Q^2 clamp 2, cubic-spline lookup 11, q^2 F / t 3, x = b q / hc 2,
+#   J1 asymptotic form: two 18-term Chebyshev series 72, 1/x and argument 6,
+#   sin/cos with argument reduction 40, combination + sqrt + divide 8,
+#   product 1, QK21 Gauss/Kronrod/abs sums 8
+FLOP_PER_QAGS_EVAL = 153
 HBM_PEAK_GBS = 8000.0
 
 
@@ -79,6 +87,9 @@ def main():
     ap.add_argument("--no-compare", action="store_true",
                     help="skip timing the other forms' grids and their error against the FP64 form")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run three 1 GiB device-to-device copies (scale factor for the "
+                         "FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the luminosity grid "
@@ -88,6 +99,16 @@ def main():
 
     import torch
     import upcgen_amd as U
+
+    if args.calibrate_copy:
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration": "copy", "bytes_read_per_copy": src.numel() * 8,
+                          "bytes_written_per_copy": dst.numel() * 8}))
+        return
 
     forms = {"fp64": U.FORM_FP64, "mfma": U.FORM_MFMA, "mfma_fp32": U.FORM_MFMA_FP32,
              "surrogate": U.FORM_SURROGATE}
@@ -126,6 +147,15 @@ def main():
     per_launch = {k: (v[0] / v[1] if v[1] else 0.0) for k, v in ph.items()}
     dominant = max(("qags", "lumi", "tables", "flux", "project"), key=lambda k: ph[k][0])
     lumi_ms = per_launch["lumi"]
+    # the dominant kernel: the QAGS form-factor flux integrals (same in every form)
+    qags_ms = per_launch["qags"]
+    qags_flop = inf.qags_evals * FLOP_PER_QAGS_EVAL
+    qags_tf = qags_flop / (qags_ms * 1e-3) / 1e12 if qags_ms > 0 else 0.0
+    roof_qags = {"bound": "mfma", "kernel": "k_flux_qags (QAGS form-factor flux, FP64 VALU)",
+                 "achieved": round(qags_tf, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                 "frac": round(qags_tf / FP64_PEAK_TFLOPS, 5), "traffic": None,
+                 "algorithmic_flop_per_launch": qags_flop, "integrand_evals_per_launch": inf.qags_evals,
+                 "flop_per_eval": FLOP_PER_QAGS_EVAL, "avg_launch_ms": round(qags_ms, 4)}
     if form == U.FORM_FP64:
         # algorithmic FLOP of the reference quadrature per launch
         flop = W_ALG * POINTS
@@ -183,7 +213,8 @@ def main():
         "config": {"workload": "Pb-Pb 5.02 TeV, Woods-Saxon form-factor flux, XNXN (mode 2), "
                                "1001x121 (M 3.56-50 GeV, Y -6..6), z x 100, PROC_ID 13",
                    "form": args.form, "grid_points": POINTS, "parallelism": "single GPU"},
-        "roofline": roof,
+        "roofline": roof_qags,
+        "roofline_contraction": roof,
         "phases_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in ph.items()},
         "dominant_phase": dominant,
         "forms": compare,
