@@ -423,7 +423,7 @@ __device__ __forceinline__ QK21Out qk21_lds(const F& f, double a, double b, doub
 }
 
 // dqpsrt on 1-based arrays.
-__device__ __noinline__ void qpsrt(int limit, int last, int* maxerr, double* ermax, const double* elist,
+__device__ __forceinline__ void qpsrt(int limit, int last, int* maxerr, double* ermax, const double* elist,
                              int* iord, int* nrmax)
 {
   int i, k, isucc, jupbn, jbnd, ibeg;
@@ -477,7 +477,7 @@ L90:
 }
 
 // dqelg (Wynn epsilon) on a 1-based table of length >= 55.
-__device__ __noinline__ void qelg(int* n, double* epstab, double* result, double* abserr, double* res3la,
+__device__ __forceinline__ void qelg(int* n, double* epstab, double* result, double* abserr, double* res3la,
                             int* nres)
 {
   constexpr double epmach = DBL_EPSILON, oflow = DBL_MAX;

@@ -20,7 +20,7 @@
 
 namespace upc {
 
-constexpr int kQagsCap = 64;       // fast per-lane interval list
+constexpr int kQagsCap = 16;       // fast per-lane interval list (the flux integrals stop by 15)
 constexpr int kQagsLimit = 1000;   // gsl_integration_qags limit of the reference
 constexpr int kQagsBigCap = 1000;
 
@@ -37,12 +37,19 @@ __device__ __forceinline__ double flux_point(double b, double k, double g1, doub
 struct FluxFormIntegrand {
   const double4* __restrict__ ff;
   double b, w2g2;
+  // The spline record is a dependent load from a 32 MB table: it is issued
+  // first and J1 (independent ALU work) runs while it is in flight.
   __device__ __forceinline__ double operator()(double q) const
   {
     const double t = q * q + w2g2;
     const double tc = t < kQ2max ? t : kQ2max - kDQ2;
-    const double f = spline_eval(ff, kQ2min, kDQ2, 1.0 / kDQ2, kNQ2, tc);
-    return q * q * f / t * bessel_j1(b * q * (1.0 / kHc));
+    double xl;
+    const int i = spline_interval(kQ2min, kDQ2, 1.0 / kDQ2, kNQ2, tc, &xl);
+    const double4 c = ff[i];
+    const double j1 = bessel_j1(b * q * (1.0 / kHc));
+    const double d = tc - xl;
+    const double f = c.x + d * (c.y + d * (c.z + d * c.w));
+    return q * q * f / t * j1;
   }
 };
 

@@ -12,11 +12,15 @@ namespace upc {
 
 enum QagsStatus { kQagsRunning = 0, kQagsDone = 1, kQagsOverflow = 2 };
 
+// CAP: interval-list capacity (1-based arrays).  The epsilon table holds at
+// most one entry per bisection (numrl2 <= last + 1) plus the two qelg works
+// past its end, and never more than dqelg's limexp = 50 (+2, 1-based).
 template <int CAP>
 struct Qags {
+  static constexpr int NEPS = CAP + 6 < 55 ? CAP + 6 : 55;
   double alist[CAP + 1], blist[CAP + 1], rlist[CAP + 1], elist[CAP + 1];
   int iord[CAP + 1];
-  double rlist2[55], res3la[4];
+  double rlist2[NEPS], res3la[4];
   double a, b, epsabs, epsrel;
   double result, abserr, area, errsum, errbnd, errmax, erlarg, ertest, small, correc, defabs,
       erlast, a1, b1, a2, b2, area1, error1, defab1;
@@ -93,7 +97,7 @@ struct Qags {
     return kQagsDone;
   }
 
-  __device__ __noinline__ int consume(const QK21Out& o)
+  __device__ __forceinline__ int consume(const QK21Out& o)
   {
     constexpr double epmach = DBL_EPSILON, uflow = DBL_MIN, oflow = DBL_MAX;
     if (phase == 0) {
@@ -214,6 +218,7 @@ struct Qags {
         nrmax++;
       }
     }
+    if (numrl2 + 3 > NEPS - 1) return kQagsOverflow;  // small tables only
     numrl2++;
     rlist2[numrl2] = area;
     double reseps, abseps;
