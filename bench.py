@@ -78,6 +78,38 @@ def cpu_baseline():
             "qags_bad": nbad}
 
 
+def batch_bench(seed, reps):
+    """64 polarised jobs (form-factor flux) in one graph-captured launch
+    sequence; the launch shapes come from the autotuner."""
+    import torch
+    import upcgen_amd as U
+    jobs = U.mixed_systems_jobs(seed=seed)
+    t0 = time.perf_counter()
+    b = U.Batch(jobs)
+    torch.cuda.synchronize()
+    t_create = time.perf_counter() - t0
+    tune = b.autotune()
+    b.run(graph=True)  # capture + warm
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(reps):
+        b.run(graph=True)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t1) * 1e3 / reps
+    st = b.stats()
+    res = {"workload": "64 jobs: Pb-Pb 5.02 TeV, Xe-Xe 5.44 TeV, O-O 7 TeV, Au-Au 200 GeV x BREAKUP_MODE 1-4, "
+                       "USE_POLARIZED_CS 1, FLUX_POINT 0, nm in {101..2001}, ny in {41, 81, 121}",
+           "seed": seed, "points": st.points, "rows": st.rows, "ms_per_batch": round(ms, 2),
+           "points_per_s": round(st.points / ms * 1e3, 1), "replays": reps,
+           "qags_calls": st.qags_calls, "qags_evals": st.qags_evals, "qags_bad": st.qags_bad,
+           "tuned": {"qags_blocks_per_cu": tune.qags_blocks_per_cu, "lumi_waves": tune.lumi_waves,
+                     "qags_ms": dict(zip(map(str, tune.qags_candidates), [round(x, 2) for x in tune.qags_ms])),
+                     "lumi_ms": dict(zip(map(str, tune.lumi_candidates), [round(x, 2) for x in tune.lumi_ms]))},
+           "create_s": round(t_create, 2)}
+    b.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -87,6 +119,10 @@ def main():
     ap.add_argument("--no-compare", action="store_true",
                     help="skip timing the other forms' grids and their error against the FP64 form")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batch", action="store_true",
+                    help="also time the 64-job mixed-systems batch (Pb/Xe/O/Au x breakup modes 1-4, "
+                         "polarised, uneven grids) as one graph-captured launch sequence")
+    ap.add_argument("--batch-seed", type=int, default=2024)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run three 1 GiB device-to-device copies (scale factor for the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -221,6 +257,8 @@ def main():
         "qags": {"calls": inf.qags_calls, "integrand_evals": inf.qags_evals, "bad": inf.qags_bad},
         "tot_cs_mb": tot,
     }
+    if args.batch:
+        out["batch"] = batch_bench(args.batch_seed, max(1, min(args.steps, 3)))
     if not args.no_cpu_baseline:
         cb = cpu_baseline()
         out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}

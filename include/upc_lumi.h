@@ -197,6 +197,48 @@ int upc_generate_pairs(upc_ctx *ctx, const double *d_sum_ym, const double *d_sum
                        unsigned long long seed, unsigned long long offset, long long n, double *d_p, int *d_pdg,
                        int *d_pass, void *stream);
 
+/* ---- batches: many configurations in one launch sequence ----
+ * No reference counterpart as one call: a batch of n jobs computes what n
+ * reference processes compute in UpcCrossSection::init + prepareTwoPhotonLumi
+ * (src/UpcCrossSection.cpp:116-137, 463-592), each job with its own system,
+ * breakup mode, flux type, polarisation and grid.  The flux integrals of all
+ * jobs share one persistent QAGS launch and the contraction is one launch;
+ * upc_batch_run(use_graph = 1) captures the sequence into a hipGraph on its
+ * first call and replays it afterwards. */
+typedef struct upc_batch upc_batch;
+
+typedef struct upc_tune {
+  int qags_blocks_per_cu;     /* chosen QAGS workgroups (256 threads) per CU */
+  int lumi_waves;             /* chosen contraction waves per workgroup */
+  int qags_candidates[3];
+  double qags_ms[3];          /* QAGS kernel time of each candidate */
+  int lumi_candidates[3];
+  double lumi_ms[3];          /* contraction kernel time of each candidate */
+} upc_tune;
+
+typedef struct upc_batch_stats {
+  int njobs;
+  long long points, rows;
+  long long qags_calls, qags_evals, qags_bad;   /* of the last run */
+  int qags_blocks_per_cu, lumi_waves;           /* launch shapes in use */
+} upc_batch_stats;
+
+/* Builds one context (tables) per job and the batch work buffers. */
+int upc_batch_create(const upc_config *cfgs, int njobs, void *stream, upc_batch **out);
+int upc_batch_destroy(upc_batch *batch);
+/* Output grids per job: d_L[j] (unpolarised jobs) or d_Ls[j], d_Lp[j]
+ * (polarised), [nm][ny] x dm x dy like upc_lumi_grid.  Arrays of n device
+ * pointers; unused entries may be NULL.  Captured graphs see new pointers. */
+int upc_batch_set_outputs(upc_batch *batch, double *const *d_L, double *const *d_Ls, double *const *d_Lp);
+/* FP64 grids of every job.  use_graph: capture once, then replay. */
+int upc_batch_run(upc_batch *batch, int use_graph, void *stream);
+/* Times every candidate launch shape on this batch and keeps the fastest
+ * (also computes the grids).  Synchronises `stream`. */
+int upc_batch_autotune(upc_batch *batch, void *stream, upc_tune *h_out);
+/* Sets the launch shapes directly (values from upc_tune's candidate lists). */
+int upc_batch_set_tune(upc_batch *batch, int qags_blocks_per_cu, int lumi_waves);
+int upc_batch_info(upc_batch *batch, void *stream, upc_batch_stats *h_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -11,7 +11,7 @@ import ctypes as C
 import os
 
 __all__ = ["Config", "Info", "Context", "load", "FORM_FP64", "FORM_MFMA", "FORM_SURROGATE", "FORM_MFMA_FP32",
-           "UpcError", "LIB_PATH"]
+           "UpcError", "LIB_PATH", "Batch", "Tune", "BatchStats", "mixed_systems_jobs", "wood_saxon_radius"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libupcgen_amd.so")
@@ -71,6 +71,18 @@ class Timings(C.Structure):
 PHASES = ("tables", "flux", "qags", "lumi", "xsec", "pdf", "project")
 
 
+class Tune(C.Structure):
+    _fields_ = [("qags_blocks_per_cu", C.c_int), ("lumi_waves", C.c_int),
+                ("qags_candidates", C.c_int * 3), ("qags_ms", C.c_double * 3),
+                ("lumi_candidates", C.c_int * 3), ("lumi_ms", C.c_double * 3)]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("njobs", C.c_int), ("points", C.c_longlong), ("rows", C.c_longlong),
+                ("qags_calls", C.c_longlong), ("qags_evals", C.c_longlong), ("qags_bad", C.c_longlong),
+                ("qags_blocks_per_cu", C.c_int), ("lumi_waves", C.c_int)]
+
+
 # Every symbol declared in include/upc_lumi.h with its ctypes signature.
 _vp, _i, _d, _ll, _ull = C.c_void_p, C.c_int, C.c_double, C.c_longlong, C.c_ulonglong
 SIGNATURES = {
@@ -100,6 +112,13 @@ SIGNATURES = {
     "upc_generate_pairs": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, C.POINTER(EventOpts), _ull, _ull, _ll, _vp,
                                 _vp, _vp, _vp]),
     "upc_sample_events": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _ull, _ull, _ll, _vp, _vp, _vp, _vp]),
+    "upc_batch_create": (_i, [C.POINTER(Config), _i, _vp, C.POINTER(_vp)]),
+    "upc_batch_destroy": (_i, [_vp]),
+    "upc_batch_set_outputs": (_i, [_vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp)]),
+    "upc_batch_run": (_i, [_vp, _i, _vp]),
+    "upc_batch_autotune": (_i, [_vp, _vp, C.POINTER(Tune)]),
+    "upc_batch_set_tune": (_i, [_vp, _i, _i]),
+    "upc_batch_info": (_i, [_vp, _vp, C.POINTER(BatchStats)]),
 }
 
 _lib = None
@@ -343,3 +362,85 @@ class Context:
                                         _dptr(sum_zp), _dptr(ratio), seed, offset, n, _dptr(y), _dptr(m),
                                         _dptr(z), _stream(stream)), "upc_sample_events")
         return y, m, z
+
+
+def wood_saxon_radius(A):
+    """R = 1.12 A^(1/3) - 0.86 A^(-1/3) fm."""
+    return 1.12 * A ** (1 / 3) - 0.86 * A ** (-1 / 3)
+
+
+def mixed_systems_jobs(seed=2024, n=64, polarized=1, flux_point=0):
+    """The 64-job mixed batch: Pb-Pb 5.02 TeV, Xe-Xe 5.44 TeV, O-O 7 TeV and
+    Au-Au 200 GeV x breakup modes 1-4 (each combination n / 16 times), M bins
+    drawn from {101, 251, 501, 1001, 2001} and Y bins from {41, 81, 121} with
+    `seed`; R from wood_saxon_radius, a = 0.54 fm except Pb (0.447)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    systems = [(82, 208, 0.447, 5020.0), (54, 129, 0.54, 5440.0), (8, 16, 0.54, 7000.0), (79, 197, 0.54, 200.0)]
+    jobs = []
+    for i in range(n):
+        Z, A, a, sqrts = systems[i % 4]
+        mode = 1 + (i // 4) % 4
+        jobs.append(Config.make(Z=Z, A=A, R=wood_saxon_radius(A), a=a, sqrts=sqrts, flux_point=flux_point,
+                                breakup_mode=mode, use_polarized=polarized,
+                                nm=int(rng.choice([101, 251, 501, 1001, 2001])),
+                                ny=int(rng.choice([41, 81, 121]))))
+    return jobs
+
+
+class Batch:
+    """Many configurations whose luminosity grids run as one launch sequence
+    (one persistent QAGS launch, one contraction launch), optionally replayed
+    from a captured hipGraph."""
+
+    def __init__(self, configs, stream=None):
+        import torch
+        if not torch.cuda.is_available():
+            raise UpcError("no HIP device visible: upcgen_amd runs on the GPU only")
+        self.configs = list(configs)
+        arr = (Config * len(self.configs))(*self.configs)
+        self._b = C.c_void_p()
+        _check(load().upc_batch_create(arr, len(self.configs), _stream(stream), C.byref(self._b)),
+               "upc_batch_create")
+        n = len(self.configs)
+        dev = "cuda"
+        self.outputs = []
+        for c in self.configs:
+            if c.use_polarized:
+                self.outputs.append((torch.empty((c.nm, c.ny), dtype=torch.float64, device=dev),
+                                     torch.empty((c.nm, c.ny), dtype=torch.float64, device=dev)))
+            else:
+                self.outputs.append(torch.empty((c.nm, c.ny), dtype=torch.float64, device=dev))
+        L = (_vp * n)(*[None if c.use_polarized else o.data_ptr() for c, o in zip(self.configs, self.outputs)])
+        Ls = (_vp * n)(*[o[0].data_ptr() if c.use_polarized else None for c, o in zip(self.configs, self.outputs)])
+        Lp = (_vp * n)(*[o[1].data_ptr() if c.use_polarized else None for c, o in zip(self.configs, self.outputs)])
+        _check(load().upc_batch_set_outputs(self._b, L, Ls, Lp), "upc_batch_set_outputs")
+
+    def close(self):
+        if self._b:
+            load().upc_batch_destroy(self._b)
+            self._b = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, graph=True, stream=None):
+        """All grids into self.outputs ([nm][ny] x dm x dy; (Ls, Lp) for polarised jobs)."""
+        _check(load().upc_batch_run(self._b, int(graph), _stream(stream)), "upc_batch_run")
+        return self.outputs
+
+    def autotune(self, stream=None) -> Tune:
+        t = Tune()
+        _check(load().upc_batch_autotune(self._b, _stream(stream), C.byref(t)), "upc_batch_autotune")
+        return t
+
+    def set_tune(self, qags_blocks_per_cu, lumi_waves):
+        _check(load().upc_batch_set_tune(self._b, qags_blocks_per_cu, lumi_waves), "upc_batch_set_tune")
+
+    def stats(self, stream=None) -> BatchStats:
+        st = BatchStats()
+        _check(load().upc_batch_info(self._b, _stream(stream), C.byref(st)), "upc_batch_info")
+        return st

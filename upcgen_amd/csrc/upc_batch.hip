@@ -1,0 +1,533 @@
+// Batched luminosity grids: many collision systems, breakup modes and uneven
+// (M, Y) grids in one launch sequence.
+//
+// Each job is one configuration, i.e. what one reference process computes in
+// UpcCrossSection::init + prepareTwoPhotonLumi (src/UpcCrossSection.cpp:
+// 116-137, 463-592).  Every job keeps its own context (tables: G_AA, form
+// factor, breakup), and the grid work of all jobs is concatenated:
+//   points [sum npts], flux rows [sum nrows][120], QAGS tasks [sum tasks].
+// One kernel per stage serves all jobs (a job table in HBM maps a point,
+// row or block to its job), so the QAGS integrals of every job share one
+// persistent launch and its task cursor, and the contraction is one launch.
+// The sequence is captured once into a hipGraph and replayed; buffers are
+// cleared by kernels (no memset nodes).  The launch shapes (QAGS workgroups
+// per CU, contraction waves per block) are chosen by timing the candidates
+// on the batch itself.
+#include "upc_flux.h"
+
+#include <algorithm>
+#include <vector>
+
+struct upc_batch;
+
+namespace upc {
+
+struct BatchJob {
+  LumiView v;          // pts, row_logd, flux point at this job's slice
+  const double4* ff;
+  int nm, ny;
+  double mmin, mmax, ymin, ymax;
+  int pt0, row0;       // first point and row of the job in the batch arrays
+};
+
+constexpr int kLumiWaveChoices[3] = {2, 4, 8};
+constexpr int kQagsBlockChoices[3] = {2, 3, 4};  // workgroups of 256 per CU
+
+__device__ __forceinline__ int find_job(const int* __restrict__ start, int njobs, long idx)
+{
+  int lo = 0, hi = njobs - 1;  // largest j with start[j] <= idx
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= idx) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ void k_batch_zero(unsigned long long* __restrict__ p, int n)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+__global__ void k_batch_points(const BatchJob* __restrict__ jobs, const int* __restrict__ pt0, int njobs,
+                               long total, double* __restrict__ pts)
+{
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= total) return;
+  const BatchJob& J = jobs[find_job(pt0, njobs, idx)];
+  const long bin = idx - J.pt0;
+  const int im = (int)(bin / J.ny), iy = (int)(bin % J.ny);
+  const double dy = (J.ymax - J.ymin) / J.ny;
+  const double dm = (J.mmax - J.mmin) / J.nm;
+  pts[4 * idx + 0] = J.mmin + dm * im;
+  pts[4 * idx + 1] = J.ymin + dy * iy;
+  pts[4 * idx + 2] = dm;
+  pts[4 * idx + 3] = dy;
+}
+
+__global__ void k_batch_rows(const BatchJob* __restrict__ jobs, const int* __restrict__ row0, int njobs, long total,
+                             double* __restrict__ row_k, double* __restrict__ row_logd, int* __restrict__ row_cnt)
+{
+  const long r = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= total) return;
+  const BatchJob& J = jobs[find_job(row0, njobs, r)];
+  row_setup(J.v.pts, J.v.npts, (int)(r - J.row0), J.v.fp, &row_k[r], &row_logd[r], &row_cnt[r]);
+}
+
+// Three-pass exclusive scan of the row counts: per-block sums, their scan in
+// one block, then the offsets.  *total receives the sum.
+constexpr int kScanBlock = 1024;
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_part(const int* __restrict__ cnt, long n,
+                                                          long long* __restrict__ part)
+{
+  __shared__ long long s[kScanBlock];
+  const long i = (long)blockIdx.x * kScanBlock + threadIdx.x;
+  s[threadIdx.x] = i < n ? cnt[i] : 0;
+  __syncthreads();
+  for (int d = kScanBlock / 2; d > 0; d >>= 1) {
+    if (threadIdx.x < d) s[threadIdx.x] += s[threadIdx.x + d];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_top(long long* __restrict__ part, int nparts,
+                                                         unsigned long long* __restrict__ total)
+{
+  __shared__ long long s[kScanBlock];
+  long long carry = 0;
+  for (int base = 0; base < nparts; base += kScanBlock) {
+    const int i = base + threadIdx.x;
+    const long long v = i < nparts ? part[i] : 0;
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int d = 1; d < kScanBlock; d <<= 1) {
+      const long long t = threadIdx.x >= d ? s[threadIdx.x - d] : 0;
+      __syncthreads();
+      s[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (i < nparts) part[i] = carry + s[threadIdx.x] - v;  // exclusive
+    const long long blk_total = s[kScanBlock - 1];
+    __syncthreads();
+    carry += blk_total;
+  }
+  if (threadIdx.x == 0) *total = (unsigned long long)carry;
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_apply(const int* __restrict__ cnt, long n,
+                                                           const long long* __restrict__ part,
+                                                           int* __restrict__ off)
+{
+  __shared__ long long s[kScanBlock];
+  const long i = (long)blockIdx.x * kScanBlock + threadIdx.x;
+  const long long v = i < n ? cnt[i] : 0;
+  s[threadIdx.x] = v;
+  __syncthreads();
+  for (int d = 1; d < kScanBlock; d <<= 1) {
+    const long long t = threadIdx.x >= d ? s[threadIdx.x - d] : 0;
+    __syncthreads();
+    s[threadIdx.x] += t;
+    __syncthreads();
+  }
+  if (i < n) off[i] = (int)(part[blockIdx.x] + s[threadIdx.x] - v);
+}
+
+__global__ void k_batch_point_rows(const BatchJob* __restrict__ jobs, const int* __restrict__ row0, int njobs,
+                                   long total_rows, const double* __restrict__ row_k,
+                                   const double* __restrict__ row_logd, const int* __restrict__ row_cnt,
+                                   const int* __restrict__ row_off, double* __restrict__ flux,
+                                   int* __restrict__ tasks)
+{
+  const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= total_rows * kNB1) return;
+  const long r = gid / kNB1;
+  const BatchJob& J = jobs[find_job(row0, njobs, r)];
+  point_row_node((int)r, (int)(gid % kNB1), J.v.fp, row_k, row_logd, row_cnt, row_off, flux, tasks);
+}
+
+// QAGS task source over all jobs: task id = global row * 128 + node.
+struct BatchTask {
+  const int* __restrict__ tasks;
+  const double* __restrict__ row_k;
+  const double* __restrict__ row_logd;
+  const BatchJob* __restrict__ jobs;
+  const int* __restrict__ row0;
+  int njobs;
+  __device__ void get(int t, FluxTask* ft) const
+  {
+    const int id = tasks[t];
+    const int r = id >> 7, j = id & 127;
+    const BatchJob& J = jobs[find_job(row0, njobs, r)];
+    double bl, bh;
+    ft->b = row_node(J.v.fp.bmin, row_logd[r], j, &bl, &bh);
+    ft->k = row_k[r];
+    ft->out = (long)r * kNB1 + j;
+    ft->ff = J.ff;
+    ft->g1 = J.v.fp.g1;
+    ft->factor = J.v.fp.factor;
+    ft->A = J.v.fp.A;
+  }
+};
+
+// Contraction job resolver: blk0[j] = first block of job j for this wave count.
+struct BatchLumi {
+  const BatchJob* __restrict__ jobs;
+  const int* __restrict__ blk0;
+  int njobs;
+  __device__ void get(int blk, int waves, LumiView* out, int* p0) const
+  {
+    const int j = find_job(blk0, njobs, blk);
+    *out = jobs[j].v;
+    *p0 = (blk - blk0[j]) * waves;
+  }
+};
+
+}  // namespace upc
+
+using namespace upc;
+
+struct upc_batch {
+  int device = 0;
+  int ncu = 256;
+  std::vector<upc_ctx*> ctx;
+  std::vector<BatchJob> jobs;  // host copy (pointers into the batch buffers)
+  long npts = 0, nrows = 0;
+  int nparts = 0;
+  int lumi_blocks[3] = {0, 0, 0};
+  BatchJob* d_jobs = nullptr;
+  int* d_pt0 = nullptr;
+  int* d_row0 = nullptr;
+  int* d_blk0[3] = {nullptr, nullptr, nullptr};
+  double* d_pts = nullptr;
+  double* d_row_k = nullptr;
+  double* d_row_logd = nullptr;
+  int* d_row_cnt = nullptr;
+  int* d_row_off = nullptr;
+  long long* d_part = nullptr;
+  double* d_flux = nullptr;
+  int* d_tasks = nullptr;
+  int* d_overflow = nullptr;
+  unsigned long long* d_counters = nullptr;  // [0] evals [1] bad [2] overflow n [4] tasks [5] cursor
+  Qags<kQagsBigCap>* d_big = nullptr;
+  int nbig = 256;
+  int qags_choice = 2, lumi_choice = 1;  // indices into the candidate lists
+  hipStream_t cap_stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int graph_q = -1, graph_l = -1;
+  bool outputs_set = false;
+};
+
+namespace {
+
+int nblk(long n, int bs) { return (int)((n + bs - 1) / bs); }
+
+int cu_count(int dev)
+{
+  int ncu = 0;
+  (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  return ncu > 0 ? ncu : 256;
+}
+
+void free_batch(upc_batch* b)
+{
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
+  void* ptrs[] = {b->d_jobs, b->d_pt0, b->d_row0, b->d_blk0[0], b->d_blk0[1], b->d_blk0[2], b->d_pts,
+                  b->d_row_k, b->d_row_logd, b->d_row_cnt, b->d_row_off, b->d_part, b->d_flux, b->d_tasks,
+                  b->d_overflow, b->d_counters, b->d_big};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  for (upc_ctx* c : b->ctx) upc_ctx_destroy(c);
+  delete b;
+}
+
+// The whole grid computation of every job (no host synchronisation).
+int enqueue(upc_batch* b, hipStream_t s)
+{
+  const int nj = (int)b->jobs.size();
+  hipLaunchKernelGGL(k_batch_zero, dim3(1), dim3(64), 0, s, b->d_counters, 8);
+  hipLaunchKernelGGL(k_batch_points, dim3(nblk(b->npts, 256)), dim3(256), 0, s, b->d_jobs, b->d_pt0, nj,
+                     b->npts, b->d_pts);
+  hipLaunchKernelGGL(k_batch_rows, dim3(nblk(b->nrows, 256)), dim3(256), 0, s, b->d_jobs, b->d_row0, nj,
+                     b->nrows, b->d_row_k, b->d_row_logd, b->d_row_cnt);
+  hipLaunchKernelGGL(k_scan_part, dim3(b->nparts), dim3(kScanBlock), 0, s, b->d_row_cnt, b->nrows, b->d_part);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanBlock), 0, s, b->d_part, b->nparts, b->d_counters + 4);
+  hipLaunchKernelGGL(k_scan_apply, dim3(b->nparts), dim3(kScanBlock), 0, s, b->d_row_cnt, b->nrows, b->d_part,
+                     b->d_row_off);
+  hipLaunchKernelGGL(k_batch_point_rows, dim3(nblk(b->nrows * kNB1, 256)), dim3(256), 0, s, b->d_jobs, b->d_row0,
+                     nj, b->nrows, b->d_row_k, b->d_row_logd, b->d_row_cnt, b->d_row_off, b->d_flux, b->d_tasks);
+  BatchTask ts{b->d_tasks, b->d_row_k, b->d_row_logd, b->d_jobs, b->d_row0, nj};
+  const int qblocks = b->ncu * kQagsBlockChoices[b->qags_choice];
+  hipLaunchKernelGGL((k_flux_qags<kQagsCap, BatchTask>), dim3(qblocks), dim3(256), 0, s, ts, b->d_counters + 4,
+                     b->d_counters + 5, b->d_flux, (int*)nullptr, b->d_counters, b->d_overflow);
+  hipLaunchKernelGGL((k_flux_qags_big<BatchTask>), dim3(b->nbig / 64), dim3(64), 0, s, ts, b->d_overflow,
+                     b->d_counters, b->d_flux, (int*)nullptr, b->d_big, b->nbig);
+  const int li = b->lumi_choice, w = kLumiWaveChoices[li];
+  BatchLumi jr{b->d_jobs, b->d_blk0[li], nj};
+  if (w == 2) hipLaunchKernelGGL((k_lumi_fp64<2, BatchLumi>), dim3(b->lumi_blocks[li]), dim3(128), 0, s, jr);
+  else if (w == 4) hipLaunchKernelGGL((k_lumi_fp64<4, BatchLumi>), dim3(b->lumi_blocks[li]), dim3(256), 0, s, jr);
+  else hipLaunchKernelGGL((k_lumi_fp64<8, BatchLumi>), dim3(b->lumi_blocks[li]), dim3(512), 0, s, jr);
+  UPC_HIP(hipGetLastError());
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int upc_batch_create(const upc_config* cfgs, int njobs, void* stream, upc_batch** out)
+{
+  if (!cfgs || njobs <= 0 || !out) return set_error("bad arguments"), UPC_EINVAL;
+  *out = nullptr;
+  hipStream_t s = (hipStream_t)stream;
+  auto* b = new upc_batch();
+  (void)hipGetDevice(&b->device);
+  b->ncu = cu_count(b->device);
+  for (int j = 0; j < njobs; ++j) {
+    upc_ctx* c = nullptr;
+    const int rc = upc_ctx_create(&cfgs[j], (void*)s, &c);
+    if (rc) {
+      free_batch(b);
+      return rc;
+    }
+    b->ctx.push_back(c);
+  }
+  // offsets
+  std::vector<int> pt0(njobs), row0(njobs), blk0[3];
+  long np = 0, nr = 0, blocks[3] = {0, 0, 0};
+  for (int j = 0; j < njobs; ++j) {
+    const upc_config& g = cfgs[j];
+    const long n = (long)g.nm * g.ny;
+    const bool mirror = g.ymin == -g.ymax;
+    pt0[j] = (int)np;
+    row0[j] = (int)nr;
+    for (int i = 0; i < 3; ++i) {
+      blk0[i].push_back((int)blocks[i]);
+      blocks[i] += nblk(n, kLumiWaveChoices[i]);
+    }
+    np += n;
+    nr += mirror ? n + g.nm : 2 * n;
+  }
+  if (nr * 128 >= (1L << 31)) {
+    free_batch(b);
+    return set_error("batch too large for 32-bit task ids"), UPC_EINVAL;
+  }
+  b->npts = np;
+  b->nrows = nr;
+  b->nparts = nblk(nr, kScanBlock);
+  for (int i = 0; i < 3; ++i) b->lumi_blocks[i] = (int)blocks[i];
+#define BA(ptr, n) UPC_HIP(hipMalloc(&(ptr), sizeof(*(ptr)) * (size_t)(n)))
+  auto alloc = [&]() -> int {
+    BA(b->d_jobs, njobs);
+    BA(b->d_pt0, njobs);
+    BA(b->d_row0, njobs);
+    for (int i = 0; i < 3; ++i) BA(b->d_blk0[i], njobs);
+    BA(b->d_pts, 4 * np);
+    BA(b->d_row_k, nr);
+    BA(b->d_row_logd, nr);
+    BA(b->d_row_cnt, nr);
+    BA(b->d_row_off, nr);
+    BA(b->d_part, b->nparts);
+    BA(b->d_flux, nr * kNB1);
+    BA(b->d_tasks, nr * kNB1);
+    BA(b->d_overflow, nr * kNB1);
+    BA(b->d_counters, 8);
+    BA(b->d_big, b->nbig);
+    return 0;
+  };
+#undef BA
+  if (int rc = alloc()) {
+    free_batch(b);
+    return rc;
+  }
+  for (int j = 0; j < njobs; ++j) {
+    const upc_ctx* c = b->ctx[j];
+    const upc_config& g = cfgs[j];
+    const bool mirror = g.ymin == -g.ymax;
+    BatchJob J{};
+    J.v.pts = b->d_pts + 4L * pt0[j];
+    J.v.row_logd = b->d_row_logd + row0[j];
+    J.v.flux = b->d_flux + (long)row0[j] * kNB1;
+    J.v.fp = flux_params(c, mirror ? g.ny : 0);
+    J.v.gaa = c->d_gaa;
+    J.v.bk = c->d_bk;
+    J.v.sc = c->d_scalars;
+    J.v.npts = g.nm * g.ny;
+    J.v.mode = g.breakup_mode;
+    J.v.pol = g.use_polarized;
+    J.ff = c->d_ff;
+    J.nm = g.nm;
+    J.ny = g.ny;
+    J.mmin = g.mmin;
+    J.mmax = g.mmax;
+    J.ymin = g.ymin;
+    J.ymax = g.ymax;
+    J.pt0 = pt0[j];
+    J.row0 = row0[j];
+    b->jobs.push_back(J);
+  }
+  auto upload = [&]() -> int {
+    UPC_HIP(hipMemcpy(b->d_jobs, b->jobs.data(), sizeof(BatchJob) * njobs, hipMemcpyHostToDevice));
+    UPC_HIP(hipMemcpy(b->d_pt0, pt0.data(), sizeof(int) * njobs, hipMemcpyHostToDevice));
+    UPC_HIP(hipMemcpy(b->d_row0, row0.data(), sizeof(int) * njobs, hipMemcpyHostToDevice));
+    for (int i = 0; i < 3; ++i)
+      UPC_HIP(hipMemcpy(b->d_blk0[i], blk0[i].data(), sizeof(int) * njobs, hipMemcpyHostToDevice));
+    UPC_HIP(hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking));
+    return 0;
+  };
+  if (int rc = upload()) {
+    free_batch(b);
+    return rc;
+  }
+  *out = b;
+  return 0;
+}
+
+int upc_batch_destroy(upc_batch* b)
+{
+  if (!b) return set_error("null batch"), UPC_EINVAL;
+  (void)hipDeviceSynchronize();
+  free_batch(b);
+  return 0;
+}
+
+int upc_batch_set_outputs(upc_batch* b, double* const* d_L, double* const* d_Ls, double* const* d_Lp)
+{
+  if (!b) return set_error("null batch"), UPC_EINVAL;
+  for (size_t j = 0; j < b->jobs.size(); ++j) {
+    BatchJob& J = b->jobs[j];
+    if (J.v.pol ? (!d_Ls || !d_Lp || !d_Ls[j] || !d_Lp[j]) : (!d_L || !d_L[j]))
+      return set_error("missing output for job " + std::to_string(j)), UPC_EINVAL;
+    J.v.L = J.v.pol ? nullptr : d_L[j];
+    J.v.Ls = J.v.pol ? d_Ls[j] : nullptr;
+    J.v.Lp = J.v.pol ? d_Lp[j] : nullptr;
+  }
+  // the job table is read by the kernels at run time: captured graphs see the new pointers
+  UPC_HIP(hipDeviceSynchronize());
+  UPC_HIP(hipMemcpy(b->d_jobs, b->jobs.data(), sizeof(BatchJob) * b->jobs.size(), hipMemcpyHostToDevice));
+  b->outputs_set = true;
+  return 0;
+}
+
+int upc_batch_run(upc_batch* b, int use_graph, void* stream)
+{
+  if (!b) return set_error("null batch"), UPC_EINVAL;
+  if (!b->outputs_set) return set_error("upc_batch_set_outputs first"), UPC_ESTATE;
+  hipStream_t s = (hipStream_t)stream;
+  if (!use_graph) return enqueue(b, s);
+  if (!b->exec || b->graph_q != b->qags_choice || b->graph_l != b->lumi_choice) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    UPC_HIP(hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue(b, b->cap_stream);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(b->cap_stream, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    UPC_HIP(e);
+    b->graph = g;
+    UPC_HIP(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+    b->graph_q = b->qags_choice;
+    b->graph_l = b->lumi_choice;
+  }
+  UPC_HIP(hipGraphLaunch(b->exec, s));
+  return 0;
+}
+
+int upc_batch_autotune(upc_batch* b, void* stream, upc_tune* h_out)
+{
+  if (!b) return set_error("null batch"), UPC_EINVAL;
+  if (!b->outputs_set) return set_error("upc_batch_set_outputs first"), UPC_ESTATE;
+  hipStream_t s = (hipStream_t)stream;
+  upc_tune t{};
+  int rc = enqueue(b, s);  // flux rows, tasks and a warm run
+  if (rc) return rc;
+  hipEvent_t e0, e1;
+  UPC_HIP(hipEventCreate(&e0));
+  UPC_HIP(hipEventCreate(&e1));
+  const int nj = (int)b->jobs.size();
+  BatchTask ts{b->d_tasks, b->d_row_k, b->d_row_logd, b->d_jobs, b->d_row0, nj};
+  for (int i = 0; i < 3; ++i) {
+    // QAGS alone: clear the counters except the task total, then time one pass
+    hipLaunchKernelGGL(k_batch_zero, dim3(1), dim3(64), 0, s, b->d_counters, 4);
+    hipLaunchKernelGGL(k_batch_zero, dim3(1), dim3(64), 0, s, b->d_counters + 5, 1);
+    UPC_HIP(hipEventRecord(e0, s));
+    const int qblocks = b->ncu * kQagsBlockChoices[i];
+    hipLaunchKernelGGL((k_flux_qags<kQagsCap, BatchTask>), dim3(qblocks), dim3(256), 0, s, ts, b->d_counters + 4,
+                       b->d_counters + 5, b->d_flux, (int*)nullptr, b->d_counters, b->d_overflow);
+    UPC_HIP(hipEventRecord(e1, s));
+    UPC_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    UPC_HIP(hipEventElapsedTime(&ms, e0, e1));
+    t.qags_candidates[i] = kQagsBlockChoices[i];
+    t.qags_ms[i] = ms;
+  }
+  // restore the overflow results (the timed passes re-ran every task)
+  hipLaunchKernelGGL((k_flux_qags_big<BatchTask>), dim3(b->nbig / 64), dim3(64), 0, s, ts, b->d_overflow,
+                     b->d_counters, b->d_flux, (int*)nullptr, b->d_big, b->nbig);
+  for (int i = 0; i < 3; ++i) {
+    BatchLumi jr{b->d_jobs, b->d_blk0[i], nj};
+    UPC_HIP(hipEventRecord(e0, s));
+    const int w = kLumiWaveChoices[i];
+    if (w == 2) hipLaunchKernelGGL((k_lumi_fp64<2, BatchLumi>), dim3(b->lumi_blocks[i]), dim3(128), 0, s, jr);
+    else if (w == 4) hipLaunchKernelGGL((k_lumi_fp64<4, BatchLumi>), dim3(b->lumi_blocks[i]), dim3(256), 0, s, jr);
+    else hipLaunchKernelGGL((k_lumi_fp64<8, BatchLumi>), dim3(b->lumi_blocks[i]), dim3(512), 0, s, jr);
+    UPC_HIP(hipEventRecord(e1, s));
+    UPC_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    UPC_HIP(hipEventElapsedTime(&ms, e0, e1));
+    t.lumi_candidates[i] = w;
+    t.lumi_ms[i] = ms;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  b->qags_choice = (int)(std::min_element(t.qags_ms, t.qags_ms + 3) - t.qags_ms);
+  b->lumi_choice = (int)(std::min_element(t.lumi_ms, t.lumi_ms + 3) - t.lumi_ms);
+  t.qags_blocks_per_cu = kQagsBlockChoices[b->qags_choice];
+  t.lumi_waves = kLumiWaveChoices[b->lumi_choice];
+  if (h_out) *h_out = t;
+  return 0;
+}
+
+int upc_batch_set_tune(upc_batch* b, int qags_blocks_per_cu, int lumi_waves)
+{
+  if (!b) return set_error("null batch"), UPC_EINVAL;
+  int qi = -1, li = -1;
+  for (int i = 0; i < 3; ++i) {
+    if (kQagsBlockChoices[i] == qags_blocks_per_cu) qi = i;
+    if (kLumiWaveChoices[i] == lumi_waves) li = i;
+  }
+  if (qi < 0 || li < 0) return set_error("unsupported launch shape"), UPC_EINVAL;
+  b->qags_choice = qi;
+  b->lumi_choice = li;
+  return 0;
+}
+
+int upc_batch_info(upc_batch* b, void* stream, upc_batch_stats* h_out)
+{
+  if (!b || !h_out) return set_error("null pointer"), UPC_EINVAL;
+  unsigned long long h[8];
+  hipStream_t s = (hipStream_t)stream;
+  UPC_HIP(hipMemcpyAsync(h, b->d_counters, sizeof(h), hipMemcpyDeviceToHost, s));
+  UPC_HIP(hipStreamSynchronize(s));
+  h_out->njobs = (int)b->jobs.size();
+  h_out->points = b->npts;
+  h_out->rows = b->nrows;
+  h_out->qags_calls = (long long)h[4];
+  h_out->qags_evals = (long long)h[0];
+  h_out->qags_bad = (long long)h[1];
+  h_out->qags_blocks_per_cu = kQagsBlockChoices[b->qags_choice];
+  h_out->lumi_waves = kLumiWaveChoices[b->lumi_choice];
+  return 0;
+}
+
+}  // extern "C"

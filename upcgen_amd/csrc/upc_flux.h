@@ -3,6 +3,9 @@
 #pragma once
 
 #include "upc_internal.h"
+#include "upc_qags.h"
+
+#include <climits>
 
 namespace upc {
 
@@ -61,6 +64,428 @@ __device__ __forceinline__ void node_k(double b1, double b2, const double* cosk,
   }
   *ks = s;
   *kp = p;
+}
+
+constexpr int kQagsCap = 16;       // fast per-lane interval list (the flux integrals stop by 15)
+constexpr int kQagsLimit = 1000;   // gsl_integration_qags limit of the reference
+constexpr int kQagsBigCap = 1000;
+
+// fluxPoint (src/UpcCrossSection.cpp:166-178): g1 for both beams.
+__device__ __forceinline__ double flux_point(double b, double k, double g1, double factor)
+{
+  const double g = g1;
+  const double x = b * k / g / kHc;
+  double K0 = 0, K1 = 0;
+  if (x > 1e-10) bessel_k01(x, &K0, &K1);
+  return factor * k / g / g * (K1 * K1 + K0 * K0 / g / g);
+}
+
+// Geometry of flux row r of a grid of npts points (src/UpcCrossSection.cpp:
+// 224-233): photon energy, log step of the b grid, and the number of its
+// nodes with b <= 2R (form-factor QAGS nodes; 0 for the point flux).
+__device__ __forceinline__ void row_setup(const double* __restrict__ pts, int npts, int r, const FluxParams& fp,
+                                          double* k_out, double* logd_out, int* cnt_out)
+{
+  int p, side;
+  if (r < npts) {
+    p = r;
+    side = 0;
+  } else {
+    p = fp.mirror_ny > 0 ? (r - npts) * fp.mirror_ny : r - npts;
+    side = 1;
+  }
+  const double M = pts[4 * p], Y = pts[4 * p + 1];
+  const double k = side ? M / 2. * exp(-Y) : M / 2. * exp(Y);
+  const double g = side ? fp.g2 : fp.g1;
+  const double bmax = fmax(5. * g * kHc / k, 5. * fp.R);
+  const double logd = (log(bmax) - log(fp.bmin)) / kNB1;
+  int cnt = 0;
+  if (!fp.isPoint) {
+    // b_j increases with j: binary search for the number of nodes with b <= 2R
+    int lo = 0, hi = kNB1;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      double bl, bh;
+      if (row_node(fp.bmin, logd, mid, &bl, &bh) > fp.twoR) hi = mid;
+      else lo = mid + 1;
+    }
+    cnt = lo;
+  }
+  *k_out = k;
+  *logd_out = logd;
+  *cnt_out = cnt;
+}
+
+// fluxFormIntegrand (src/UpcCrossSection.cpp:181-191):
+//   q^2 F(min(t, Q2max - dQ2)) / t * J1(b q / hc),  t = q^2 + (k / g1)^2
+struct FluxFormIntegrand {
+  const double4* __restrict__ ff;
+  double b, w2g2;
+  // The spline record is a dependent load from a 32 MB table: it is issued
+  // first and J1 (independent ALU work) runs while it is in flight.
+  __device__ __forceinline__ double operator()(double q) const
+  {
+    const double t = q * q + w2g2;
+    const double tc = t < kQ2max ? t : kQ2max - kDQ2;
+    double xl;
+    const int i = spline_interval(kQ2min, kDQ2, 1.0 / kDQ2, kNQ2, tc, &xl);
+    const double4 c = ff[i];
+    const double j1 = bessel_j1(b * q * (1.0 / kHc));
+    const double d = tc - xl;
+    const double f = c.x + d * (c.y + d * (c.z + d * c.w));
+    return q * q * f / t * j1;
+  }
+};
+
+// Node j of flux row r: the point-charge flux (b > 2R or FLUX_POINT 1), or
+// a QAGS task id r * 128 + j at the row's task offset.
+__device__ __forceinline__ void point_row_node(int r, int j, const FluxParams& fp, const double* __restrict__ row_k,
+                                               const double* __restrict__ row_logd,
+                                               const int* __restrict__ row_cnt, const int* __restrict__ row_off,
+                                               double* __restrict__ flux, int* __restrict__ tasks)
+{
+  if (j < row_cnt[r]) {
+    tasks[row_off[r] + j] = r * 128 + j;
+    return;
+  }
+  double bl, bh;
+  const double b = row_node(fp.bmin, row_logd[r], j, &bl, &bh);
+  flux[(long)r * kNB1 + j] = flux_point(b, row_k[r], fp.g1, fp.factor);
+}
+
+// One form-factor flux integral (fluxForm, src/UpcCrossSection.cpp:194-218):
+// flux[out] = factor (I / A)^2 / k.
+struct FluxTask {
+  double b, k;
+  long out;
+  const double4* ff;  // form-factor spline of the task's nucleus
+  double g1, factor;
+  int A;
+};
+
+// Task sources of the QAGS kernel: get(t) describes task t.
+struct GridTask {  // flux-row nodes of one context: id = row * 128 + j
+  const int* __restrict__ tasks;
+  const double* __restrict__ row_k;
+  const double* __restrict__ row_logd;
+  FluxParams fp;
+  const double4* ff;
+  __device__ void get(int t, FluxTask* ft) const
+  {
+    const int id = tasks[t];
+    const int r = id >> 7, j = id & 127;
+    double bl, bh;
+    ft->b = row_node(fp.bmin, row_logd[r], j, &bl, &bh);
+    ft->k = row_k[r];
+    ft->out = (long)r * kNB1 + j;
+    ft->ff = ff;
+    ft->g1 = fp.g1;
+    ft->factor = fp.factor;
+    ft->A = fp.A;
+  }
+};
+
+struct PointTask {  // arbitrary (b, k) points of one context
+  const double* __restrict__ b;
+  const double* __restrict__ k;
+  const int* __restrict__ idx;  // compacted indices of points needing QAGS
+  FluxParams fp;
+  const double4* ff;
+  __device__ void get(int t, FluxTask* ft) const
+  {
+    const int i = idx[t];
+    ft->b = b[i];
+    ft->k = k[i];
+    ft->out = i;
+    ft->ff = ff;
+    ft->g1 = fp.g1;
+    ft->factor = fp.factor;
+    ft->A = fp.A;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Form-factor flux: persistent QAGS kernel.  Each lane owns one integral at a
+// time; lanes take new tasks from a global cursor as they finish, so the
+// wavefront keeps issuing full-width QK21 passes.  stats: [0] integrand
+// evaluations, [1] integrals with ier != 0, [2] overflow count.
+// ---------------------------------------------------------------------------
+template <int CAP, class TS>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_flux_qags(
+    TS ts, const unsigned long long* __restrict__ ntasks_p, unsigned long long* __restrict__ cursor,
+    double* __restrict__ flux, int* __restrict__ ier_out, unsigned long long* __restrict__ stats,
+    int* __restrict__ overflow)
+{
+  __shared__ double fvbuf[20 * 256];
+  Qags<CAP> q;
+  const long ntasks = (long)*ntasks_p;
+  const int lane = threadIdx.x & 63;
+  double* fv = fvbuf + (threadIdx.x >> 6) * 20 * 64 + lane;
+  long task = -1;  // -1: needs a task; LONG_MAX: exhausted
+  FluxTask ft{};
+  FluxFormIntegrand f;
+  unsigned long long my_evals = 0, my_bad = 0;
+  while (true) {
+    const bool need = task < 0;
+    const unsigned long long mask = __ballot(need);
+    if (mask) {
+      const int leader = __ffsll((long long)mask) - 1;
+      unsigned long long base = 0;
+      if (lane == leader) base = atomicAdd(cursor, (unsigned long long)__popcll(mask));
+      base = __shfl(base, leader);
+      if (need) {
+        const unsigned long long rank = __popcll(mask & ((1ull << lane) - 1ull));
+        const long t = (long)(base + rank);
+        if (t < ntasks) {
+          task = t;
+          ts.get((int)t, &ft);
+          f.ff = ft.ff;
+          f.b = ft.b;
+          f.w2g2 = ft.k * ft.k / ft.g1 / ft.g1;
+          q.start(0., 10., 1e-4, 1e-4, kQagsLimit);
+        } else {
+          task = LONG_MAX;
+        }
+      }
+    }
+    if (__ballot(task != LONG_MAX) == 0) break;
+    if (task != LONG_MAX) {
+      double qa, qb;
+      q.interval(&qa, &qb);
+      const QK21Out o = qk21_lds(f, qa, qb, fv, 64);
+      const int st = q.consume(o);
+      if (st == kQagsDone) {
+        const double Q = q.result / ft.A;
+        flux[ft.out] = ft.factor * Q * Q / ft.k;
+        if (ier_out) ier_out[ft.out] = q.ier;
+        my_evals += (unsigned long long)q.neval();
+        my_bad += q.ier != 0;
+        task = -1;
+      } else if (st == kQagsOverflow) {
+        const unsigned long long slot = atomicAdd(&stats[2], 1ull);
+        overflow[slot] = (int)task;
+        task = -1;
+      }
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    my_evals += __shfl_down(my_evals, o);
+    my_bad += __shfl_down(my_bad, o);
+  }
+  if (lane == 0) {
+    atomicAdd(&stats[0], my_evals);
+    atomicAdd(&stats[1], my_bad);
+  }
+}
+
+// Tasks that outgrew the fast interval list: rerun with the reference's
+// 1000-interval workspace, one per lane, workspace in global memory.
+template <class TS>
+__global__ __launch_bounds__(64) void k_flux_qags_big(TS ts, const int* __restrict__ overflow,
+                                                      unsigned long long* __restrict__ stats,
+                                                      double* __restrict__ flux, int* __restrict__ ier_out,
+                                                      Qags<kQagsBigCap>* __restrict__ ws, int nws)
+{
+  const long n = (long)stats[2];
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= nws) return;
+  __shared__ double fvbuf[20 * 64];
+  double* fv = fvbuf + threadIdx.x;
+  Qags<kQagsBigCap>& q = ws[gid];
+  FluxFormIntegrand f;
+  for (long i = gid; i < n; i += nws) {
+    FluxTask ft;
+    ts.get(overflow[i], &ft);
+    f.ff = ft.ff;
+    f.b = ft.b;
+    f.w2g2 = ft.k * ft.k / ft.g1 / ft.g1;
+    q.start(0., 10., 1e-4, 1e-4, kQagsLimit);
+    int st = kQagsRunning;
+    while (st == kQagsRunning) {
+      double qa, qb;
+      q.interval(&qa, &qb);
+      st = q.consume(qk21_lds(f, qa, qb, fv, 64));
+    }
+    const double Q = q.result / ft.A;
+    flux[ft.out] = ft.factor * Q * Q / ft.k;
+    if (ier_out) ier_out[ft.out] = st == kQagsDone ? q.ier : 99;
+    atomicAdd(&stats[0], (unsigned long long)q.neval());
+    if (st != kQagsDone || q.ier != 0) atomicAdd(&stats[1], 1ull);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// FP64 luminosity: one wavefront per (M, Y) point.
+//
+// For a pair of nodes (b1_i, b2_j) the phi sum K_ij = sum_k w_k P(b) G(b),
+// b^2 = b1^2 + b2^2 +- 2 b1 b2 cos(phi_k).  Where every node has b >= 20 fm
+// the reference evaluates P at the 20 fm clamp and G = 1, so K_ij is the
+// constant S0 (same arithmetic).  For fixed i the pairs with some b < 20 form
+// one contiguous j range (the smallest b^2 is always the extreme-cos node), so
+//   L = 2 pi^2 M [ S0 (sum_i W1_i)(sum_j W2_j) + sum_{band} W1_i W2_j (K_ij - S0) ]
+// with W = N(b, k) b (b_h - b_l).  The band pairs are spread evenly over the
+// 64 lanes.
+// ---------------------------------------------------------------------------
+
+// What the contraction reads of one job (one context's grid).
+struct LumiView {
+  const double* pts;       // (M, Y, dm, dy) of the job's points
+  const double* row_logd;  // the job's flux rows (k2 rows after the k1 rows)
+  const double* flux;
+  FluxParams fp;
+  const double4* gaa;
+  const double4* bk;
+  const DevScalars* sc;
+  int npts, mode, pol;
+  double* L;
+  double* Ls;
+  double* Lp;
+};
+
+// Job resolvers: the job and first point of a block.
+struct SingleLumi {
+  LumiView v;
+  __device__ void get(int blk, int waves, LumiView* out, int* p0) const
+  {
+    *out = v;
+    *p0 = blk * waves;
+  }
+};
+
+template <int WAVES>
+struct LumiSmem {
+  double4 gaa[kNB];
+  double sb1[WAVES][kNB1], sb2[WAVES][kNB2];
+  double sw1[WAVES][kNB1], sw2[WAVES][kNB2];
+  int spre[WAVES][kNB1 + 1], sjlo[WAVES][kNB1];
+  double cosk[5], w5[10];
+};
+
+template <bool POL, int WAVES>
+__device__ __forceinline__ void lumi_fp64_body(const LumiView& v, int p0, LumiSmem<WAVES>& sm)
+{
+  for (int i = threadIdx.x; i < kNB - 1; i += blockDim.x) sm.gaa[i] = v.gaa[i];
+  if (threadIdx.x < 5) {
+    const double c = cos(kPi * kGLx5[threadIdx.x]);
+    const double sn = sin(kPi * kGLx5[threadIdx.x]);
+    sm.cosk[threadIdx.x] = c;
+    sm.w5[threadIdx.x] = kGLw5[threadIdx.x];
+    sm.w5[5 + threadIdx.x] = sn * sn;
+  }
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int p = p0 + wv;
+  const bool live = p < v.npts;
+  const FluxParams& fp = v.fp;
+  const double p20 = v.sc->p20;
+  double M = 0, dm = 1, dy = 1;
+  if (live) {
+    M = v.pts[4 * p];
+    dm = v.pts[4 * p + 2];
+    dy = v.pts[4 * p + 3];
+    const int r2 = k2_row(p, v.npts, fp.mirror_ny);
+    const double logd1 = v.row_logd[p], logd2 = v.row_logd[r2];
+    const double* f1 = v.flux + (long)p * kNB1;
+    const double* f2 = v.flux + (long)r2 * kNB1;
+    for (int j = lane; j < kNB1; j += 64) {
+      double bl, bh;
+      const double b1 = row_node(fp.bmin, logd1, j, &bl, &bh);
+      sm.sb1[wv][j] = b1;
+      sm.sw1[wv][j] = f1[j] * b1 * (bh - bl);
+      const double b2 = row_node(fp.bmin, logd2, j, &bl, &bh);
+      sm.sb2[wv][j] = b2;
+      sm.sw2[wv][j] = f2[j] * b2 * (bh - bl);
+    }
+  }
+  __syncthreads();
+  // the extreme-cos node has the smallest b for every pair
+  const double cext = POL ? -sm.cosk[0] : sm.cosk[4];
+  if (live) {
+    for (int i = lane; i < kNB1; i += 64) {
+      const double b1 = sm.sb1[wv][i];
+      int lo = -1, hi = -1;
+      for (int j = 0; j < kNB2; ++j) {
+        if (node_b2(b1, sm.sb2[wv][j], cext) < 400.0) {
+          if (lo < 0) lo = j;
+          hi = j;
+        }
+      }
+      sm.sjlo[wv][i] = lo < 0 ? 0 : lo;
+      sm.spre[wv][i + 1] = lo < 0 ? 0 : hi - lo + 1;
+    }
+  }
+  __syncthreads();
+  if (live && lane == 0) {
+    sm.spre[wv][0] = 0;
+    for (int i = 1; i <= kNB1; ++i) sm.spre[wv][i] += sm.spre[wv][i - 1];
+  }
+  __syncthreads();
+  if (!live) return;
+  // S0: the phi sum when every node is at or beyond 20 fm
+  double s0s = 0, s0p = 0;
+  {
+    const double br = v.mode != 1 ? p20 : 1.;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      if (POL) {
+        s0s += br * 1. * sm.w5[k] * sm.cosk[k] * sm.cosk[k];
+        s0p += br * 1. * sm.w5[k] * sm.w5[5 + k];
+      } else {
+        s0s += br * 1. * sm.w5[k];
+      }
+    }
+  }
+  const int T = sm.spre[wv][kNB1];
+  double accs = 0, accp = 0;
+  for (int t = lane; t < T; t += 64) {
+    int lo = 0, hi = kNB1;  // largest i with pre[i] <= t
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (sm.spre[wv][mid] <= t) lo = mid;
+      else hi = mid;
+    }
+    const int i = lo;
+    const int j = sm.sjlo[wv][i] + (t - sm.spre[wv][i]);
+    double ks, kp;
+    node_k<POL>(sm.sb1[wv][i], sm.sb2[wv][j], sm.cosk, sm.w5, v.mode, p20, sm.gaa, v.bk, &ks, &kp);
+    const double ww = sm.sw1[wv][i] * sm.sw2[wv][j];
+    accs += ww * (ks - s0s);
+    if (POL) accp += ww * (kp - s0p);
+  }
+  double sw1t = 0, sw2t = 0;
+  for (int j = lane; j < kNB1; j += 64) {
+    sw1t += sm.sw1[wv][j];
+    sw2t += sm.sw2[wv][j];
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    accs += __shfl_down(accs, o);
+    accp += __shfl_down(accp, o);
+    sw1t += __shfl_down(sw1t, o);
+    sw2t += __shfl_down(sw2t, o);
+  }
+  if (lane == 0) {
+    const double pre = 2 * kPi * kPi * M;
+    const double tot_s = s0s * (sw1t * sw2t) + accs;
+    if (POL) {
+      const double tot_p = s0p * (sw1t * sw2t) + accp;
+      v.Ls[p] = pre * tot_s * dm * dy;
+      v.Lp[p] = pre * tot_p * dm * dy;
+    } else {
+      v.L[p] = pre * tot_s * dm * dy;
+    }
+  }
+}
+
+// WAVES points per block; the resolver maps the block to its job (all
+// threads of a block take the same branch, so the barriers are uniform).
+template <int WAVES, class JR>
+__global__ __launch_bounds__(64 * WAVES) void k_lumi_fp64(JR jr)
+{
+  __shared__ LumiSmem<WAVES> sm;
+  LumiView v;
+  int p0;
+  jr.get(blockIdx.x, WAVES, &v, &p0);
+  if (v.pol) lumi_fp64_body<true, WAVES>(v, p0, sm);
+  else lumi_fp64_body<false, WAVES>(v, p0, sm);
 }
 
 inline FluxParams flux_params(const upc_ctx* c, int mirror_ny = 0)
