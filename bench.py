@@ -110,6 +110,63 @@ def batch_bench(seed, reps):
     return res
 
 
+def surrogate_bench(ctx, L, save_path):
+    """Train on 2^20 drawn points (512 systems x 2048; about half fall in the
+    surrogate's domain), evaluate on 2^16 held-out points (32 new systems x
+    2048), then serve this context's grid."""
+    import numpy as np
+    import torch
+    import upcgen_amd as U
+    from upcgen_amd import surrogate as S
+    t0 = time.perf_counter()
+    X, r, _, _ = S.make_dataset(512, 2048, seed=11)
+    torch.cuda.synchronize()
+    t_data = time.perf_counter() - t0
+    mu, sig = float(r.mean()), float(r.std())
+    s = S.Surrogate(11)
+    s.norm = (mu, sig)
+    t1 = time.perf_counter()
+    steps = 20000
+    loss = s.train(X.contiguous(), ((r - mu) / sig).float().contiguous(), steps=steps, batch=4096, lr=2e-3)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t1
+    Xv, _, lgv, L0v = S.make_dataset(0, 2048, 16, systems=S.sample_systems(32, 18))
+    scale = s.calibrate(Xv.contiguous(), L0v, lgv)
+    Xh, _, lgh, L0h = S.make_dataset(0, 2048, 99, systems=S.sample_systems(32, 1234))
+    pred, spread = s.log_lumi(Xh.contiguous(), L0h)
+    err = (pred - lgh).abs().cpu().numpy()
+    base = (L0h + mu - lgh).abs().cpu().numpy()
+    pe = (scale * spread).cpu().numpy()
+    acc = pe <= 1e-3
+    res = {"train_rows": int(X.shape[0]), "heldout_rows": int(Xh.shape[0]), "steps": steps, "batch": 4096,
+           "data_s": round(t_data, 2), "train_s": round(t_train, 2),
+           "loss_first_last": [float(loss[:, :50].mean()), float(loss[:, -50:].mean())],
+           "err_scale": round(scale, 2),
+           "heldout_abs_dlogL": {"median": float(np.median(err)), "p90": float(np.quantile(err, 0.9)),
+                                 "p99": float(np.quantile(err, 0.99))},
+           "baseline_abs_dlogL_median": float(np.median(base)),
+           "heldout_served_frac_at_1e-3": float(acc.mean()),
+           "heldout_served_max_err": float(err[acc].max()) if acc.any() else None,
+           "heldout_served_frac_within_1e-3": float((err[acc] <= 1e-3).mean()) if acc.any() else None}
+    # serve the workload grid
+    ref = torch.empty_like(L)
+    ctx.lumi_grid(U.FORM_FP64, out=ref)
+    s.attach(ctx, max_err=1e-3)
+    ctx.lumi_grid(U.FORM_SURROGATE, out=L)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    ctx.lumi_grid(U.FORM_SURROGATE, out=L)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t2) * 1e3
+    inf = ctx.info
+    e = ((L - ref).abs() / ref).flatten()
+    res["grid"] = {"ms_per_grid": round(ms, 3), "served": inf.surrogate_points, "fallback": inf.surrogate_fallback,
+                   "max_rel_err_vs_fp64": float(e.max()), "median_rel_err_vs_fp64": float(e.median())}
+    if save_path:
+        s.save(save_path)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -123,6 +180,10 @@ def main():
                     help="also time the 64-job mixed-systems batch (Pb/Xe/O/Au x breakup modes 1-4, "
                          "polarised, uneven grids) as one graph-captured launch sequence")
     ap.add_argument("--batch-seed", type=int, default=2024)
+    ap.add_argument("--surrogate", action="store_true",
+                    help="also train the surrogate on 2^20 quadrature points, evaluate it on 2^16 held-out "
+                         "points and serve the grid with it (UPC_FORM_SURROGATE, 1e-3 fallback)")
+    ap.add_argument("--surrogate-out", default=None, help="save the trained surrogate (.npz)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run three 1 GiB device-to-device copies (scale factor for the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -259,6 +320,8 @@ def main():
     }
     if args.batch:
         out["batch"] = batch_bench(args.batch_seed, max(1, min(args.steps, 3)))
+    if args.surrogate:
+        out["surrogate"] = surrogate_bench(ctx, L, args.surrogate_out)
     if not args.no_cpu_baseline:
         cb = cpu_baseline()
         out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}

@@ -69,6 +69,8 @@ typedef struct upc_info {
   long long qags_evals;  /* integrand evaluations of the last grid */
   long long mfma_ksteps; /* 16-point x 16-column x 32-deep MFMA tile steps of the last MFMA grid */
   double mfma_spacing;   /* shared b-grid spacing of the MFMA forms [fm] */
+  long long surrogate_points;    /* bins of the last surrogate grid served by the network */
+  long long surrogate_fallback;  /* bins of the last surrogate grid recomputed by quadrature */
 } upc_info;
 
 /* Phases timed with HIP events when timing is enabled (upc_ctx_set_timing). */
@@ -238,6 +240,47 @@ int upc_batch_autotune(upc_batch *batch, void *stream, upc_tune *h_out);
 /* Sets the launch shapes directly (values from upc_tune's candidate lists). */
 int upc_batch_set_tune(upc_batch *batch, int qags_blocks_per_cu, int lumi_waves);
 int upc_batch_info(upc_batch *batch, void *stream, upc_batch_stats *h_out);
+
+/* ---- learned surrogate of log L(M, Y; system) ----
+ * No reference counterpart (the reference always integrates).  Two MLPs
+ * (inputs: log M, Y, log gamma, Z, A^(1/3), R, a, one-hot breakup mode,
+ * log xi1, log xi2 with xi = k R / (gamma hc); 3 x
+ * 128 SiLU; bf16 operands, FP32 accumulation on MFMA) predict
+ * r = log L - log L0 with L0 = (M/2) n(k1) n(k2) the analytic point-charge
+ * luminosity beyond b = R; targets are normalised (r - mu) / sigma.
+ * Trained by the library's own forward/backward/Adam kernels. */
+typedef struct upc_sur upc_sur;
+int upc_sur_create(unsigned long long seed, upc_sur **out);
+int upc_sur_destroy(upc_sur *sur);
+int upc_sur_param_count(int *h_n);   /* floats per network */
+int upc_sur_get_params(upc_sur *sur, int net, float *h_out);
+int upc_sur_set_params(upc_sur *sur, int net, const float *h_in);
+int upc_sur_set_norm(upc_sur *sur, double mu, double sigma);
+int upc_sur_get_norm(upc_sur *sur, double *h_mu, double *h_sigma);
+/* Predicted error of a bin = scale * |r_a - r_b| (the ensemble spread scaled
+ * by a factor calibrated on held-out data; default 1). */
+int upc_sur_set_err_scale(upc_sur *sur, double scale);
+int upc_sur_get_err_scale(upc_sur *sur, double *h_scale);
+/* Features [n][32] (FP32) and log L0 of n (M, Y) points of one system.
+ * log L0 is NaN outside the surrogate's domain, max(xi1, xi2) <= 4 with
+ * xi = k R / (gamma hc): there L is exponentially small and the QAGS values
+ * are mostly quadrature error, so such bins are always integrated. */
+int upc_sur_features(const upc_config *cfg, const double *d_M, const double *d_Y, long long n, float *d_X,
+                     double *d_logL0, void *stream);
+/* Normalised prediction of network `net` for B rows (B a multiple of 32, <= 65536). */
+int upc_sur_forward(upc_sur *sur, int net, const float *d_X, int B, float *d_pred, void *stream);
+/* Gradient of mean((pred - t)^2) w.r.t. all parameters of `net` (param_count floats). */
+int upc_sur_grad(upc_sur *sur, int net, const float *d_X, const float *d_T, int B, float *d_grad, void *stream);
+/* Adam training of both networks on n rows (mini-batches drawn with Philox);
+ * use_graph: one captured step replayed `steps` times.  h_loss (optional):
+ * [2][steps] mean squared error per step. */
+int upc_sur_train(upc_sur *sur, const float *d_X, const float *d_T, long long n, int steps, int batch, double lr,
+                  int use_graph, void *stream, float *h_loss);
+/* Attach a trained surrogate to a context for upc_lumi_grid(UPC_FORM_SURROGATE):
+ * bins whose predicted error exceeds max_err (in log L) are
+ * recomputed by the FP64 quadrature (the spread times the surrogate's error
+ * scale), as are bins outside its domain.  sur = NULL detaches. */
+int upc_ctx_set_surrogate(upc_ctx *ctx, upc_sur *sur, double max_err);
 
 #ifdef __cplusplus
 }

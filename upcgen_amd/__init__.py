@@ -57,7 +57,8 @@ class Info(C.Structure):
     _fields_ = [("g1", C.c_double), ("g2", C.c_double), ("factor", C.c_double),
                 ("mNucl", C.c_double), ("rho0", C.c_double), ("csNN", C.c_double),
                 ("p20", C.c_double), ("qags_calls", C.c_longlong), ("qags_bad", C.c_longlong),
-                ("qags_evals", C.c_longlong), ("mfma_ksteps", C.c_longlong), ("mfma_spacing", C.c_double)]
+                ("qags_evals", C.c_longlong), ("mfma_ksteps", C.c_longlong), ("mfma_spacing", C.c_double),
+                ("surrogate_points", C.c_longlong), ("surrogate_fallback", C.c_longlong)]
 
 
 class EventOpts(C.Structure):
@@ -119,6 +120,20 @@ SIGNATURES = {
     "upc_batch_autotune": (_i, [_vp, _vp, C.POINTER(Tune)]),
     "upc_batch_set_tune": (_i, [_vp, _i, _i]),
     "upc_batch_info": (_i, [_vp, _vp, C.POINTER(BatchStats)]),
+    "upc_sur_create": (_i, [_ull, C.POINTER(_vp)]),
+    "upc_sur_destroy": (_i, [_vp]),
+    "upc_sur_param_count": (_i, [C.POINTER(_i)]),
+    "upc_sur_get_params": (_i, [_vp, _i, _vp]),
+    "upc_sur_set_params": (_i, [_vp, _i, _vp]),
+    "upc_sur_set_norm": (_i, [_vp, _d, _d]),
+    "upc_sur_get_norm": (_i, [_vp, C.POINTER(_d), C.POINTER(_d)]),
+    "upc_sur_set_err_scale": (_i, [_vp, _d]),
+    "upc_sur_get_err_scale": (_i, [_vp, C.POINTER(_d)]),
+    "upc_sur_features": (_i, [C.POINTER(Config), _vp, _vp, _ll, _vp, _vp, _vp]),
+    "upc_sur_forward": (_i, [_vp, _i, _vp, _i, _vp, _vp]),
+    "upc_sur_grad": (_i, [_vp, _i, _vp, _vp, _i, _vp, _vp]),
+    "upc_sur_train": (_i, [_vp, _vp, _vp, _ll, _i, _i, _d, _i, _vp, _vp]),
+    "upc_ctx_set_surrogate": (_i, [_vp, _vp, _d]),
 }
 
 _lib = None

@@ -42,6 +42,7 @@ int lumi_grid_mfma(upc_ctx* c, int prec, double* d_L, double* d_Ls, double* d_Lp
 int mfma_set_spacing(upc_ctx* c, double h);
 unsigned long long mfma_last_ksteps(upc_ctx* c);
 double mfma_spacing(upc_ctx* c);
+void surrogate_counts(upc_ctx* c, long long* served, long long* fallback);
 int lumi_grid_surrogate(upc_ctx* c, double* d_L, double* d_Ls, double* d_Lp, hipStream_t s);
 void free_mfma(upc_ctx* c);
 void free_surrogate(upc_ctx* c);
@@ -148,6 +149,7 @@ int upc_ctx_info(upc_ctx* c, upc_info* h)
   h->qags_evals = c->last_qags_evals;
   h->mfma_ksteps = (long long)mfma_last_ksteps(c);
   h->mfma_spacing = mfma_spacing(c);
+  surrogate_counts(c, &h->surrogate_points, &h->surrogate_fallback);
   return 0;
 }
 

@@ -185,6 +185,31 @@ __device__ __forceinline__ void bessel_k01(double x, double* k0, double* k1)
   *k1 = s1 * e;
 }
 
+// e^x K0(x), e^x K1(x): the same series, without the exponential underflow.
+__device__ __forceinline__ void bessel_k01_scaled(double x, double* k0s, double* k1s)
+{
+  if (x <= 1.0) {
+    double k0, k1;
+    bessel_k01(x, &k0, &k1);
+    const double e = exp(x);
+    *k0s = k0 * e;
+    *k1s = k1 * e;
+    return;
+  }
+  const double t = 1.0 / x;
+  double s0, s1;
+  if (x <= 4.0) {
+    const double u = (2.0 * t - 1.25) * (1.0 / 0.75);
+    clenshaw2(kK0A, kK1A, u, &s0, &s1);
+  } else {
+    const double u = 8.0 * t - 1.0;
+    clenshaw2(kK0B, kK1B, u, &s0, &s1);
+  }
+  const double e = sqrt(t);
+  *k0s = s0 * e;
+  *k1s = s1 * e;
+}
+
 // Abramowitz & Stegun 9.8.3/9.8.4: the polynomial I1 of TMath::BesselI1.
 __device__ __forceinline__ double bessel_i1_as(double x)
 {

@@ -1,0 +1,218 @@
+"""Learned surrogate of log L(M, Y; system): data generation by the FP64 GPU
+quadrature, training by the library's own HIP kernels, and serving as the
+UPC_FORM_SURROGATE luminosity form (with quadrature fallback).
+
+Training distribution (one system per context, points drawn per system):
+Z ~ U[8, 92], A = round(2.5 Z), sqrt(s_NN) log-uniform in [0.2, 40] TeV,
+breakup mode uniform in 1..4, form-factor flux, Woods-Saxon R = (1.12 A^(1/3)
+- 0.86 A^(-1/3)) x U[0.95, 1.05] fm and a ~ U[0.42, 0.58] fm (so that the
+Pb-Pb R = 6.68 fm, a = 0.447 fm system lies inside it); M log-uniform in
+[1, 200] GeV, Y ~ U[-7, 7].
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import Config, Context, UpcError, _check, _dptr, _stream, load, wood_saxon_radius
+
+IN = 32
+
+
+def _dev():
+    import torch
+    if not torch.cuda.is_available():
+        raise UpcError("no HIP device visible: upcgen_amd runs on the GPU only")
+    return torch
+
+
+def param_count() -> int:
+    n = C.c_int()
+    _check(load().upc_sur_param_count(C.byref(n)), "upc_sur_param_count")
+    return n.value
+
+
+def features(config: Config, M, Y, stream=None):
+    """(X [n][32] float32, log L0 [n] float64) on the GPU for points of one system."""
+    torch = _dev()
+    M = torch.as_tensor(M, dtype=torch.float64, device="cuda").contiguous()
+    Y = torch.as_tensor(Y, dtype=torch.float64, device="cuda").contiguous()
+    n = M.numel()
+    X = torch.empty((n, IN), dtype=torch.float32, device="cuda")
+    L0 = torch.empty(n, dtype=torch.float64, device="cuda")
+    _check(load().upc_sur_features(C.byref(config), _dptr(M), _dptr(Y), n, _dptr(X), _dptr(L0), _stream(stream)),
+           "upc_sur_features")
+    return X, L0
+
+
+class Surrogate:
+    """Two-network ensemble (upc_sur)."""
+
+    def __init__(self, seed: int = 1):
+        _dev()
+        self._s = C.c_void_p()
+        _check(load().upc_sur_create(seed, C.byref(self._s)), "upc_sur_create")
+        self.nparams = param_count()
+
+    def close(self):
+        if self._s:
+            load().upc_sur_destroy(self._s)
+            self._s = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_params(self, net: int) -> np.ndarray:
+        out = np.empty(self.nparams, dtype=np.float32)
+        _check(load().upc_sur_get_params(self._s, net, out.ctypes.data_as(C.c_void_p)), "upc_sur_get_params")
+        return out
+
+    def set_params(self, net: int, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        if p.size != self.nparams:
+            raise UpcError(f"expected {self.nparams} parameters")
+        _check(load().upc_sur_set_params(self._s, net, p.ctypes.data_as(C.c_void_p)), "upc_sur_set_params")
+
+    @property
+    def norm(self):
+        mu, sig = C.c_double(), C.c_double()
+        _check(load().upc_sur_get_norm(self._s, C.byref(mu), C.byref(sig)), "upc_sur_get_norm")
+        return mu.value, sig.value
+
+    @norm.setter
+    def norm(self, v):
+        _check(load().upc_sur_set_norm(self._s, float(v[0]), float(v[1])), "upc_sur_set_norm")
+
+    @property
+    def err_scale(self):
+        v = C.c_double()
+        _check(load().upc_sur_get_err_scale(self._s, C.byref(v)), "upc_sur_get_err_scale")
+        return v.value
+
+    @err_scale.setter
+    def err_scale(self, v):
+        _check(load().upc_sur_set_err_scale(self._s, float(v)), "upc_sur_set_err_scale")
+
+    def calibrate(self, X, logL0, log_true, quantile=0.99):
+        """Error scale from held-out points: the given quantile of |error| / spread."""
+        pred, spread = self.log_lumi(X, logL0)
+        ratio = ((pred - log_true).abs() / spread.clamp_min(1e-12)).cpu().numpy()
+        self.err_scale = float(max(1.0, np.quantile(ratio, quantile)))
+        return self.err_scale
+
+    def forward(self, net: int, X, stream=None):
+        torch = _dev()
+        B = X.shape[0]
+        out = torch.empty(B, dtype=torch.float32, device="cuda")
+        _check(load().upc_sur_forward(self._s, net, _dptr(X), B, _dptr(out), _stream(stream)), "upc_sur_forward")
+        return out
+
+    def grad(self, net: int, X, T, stream=None):
+        torch = _dev()
+        g = torch.empty(self.nparams, dtype=torch.float32, device="cuda")
+        _check(load().upc_sur_grad(self._s, net, _dptr(X), _dptr(T), X.shape[0], _dptr(g), _stream(stream)),
+               "upc_sur_grad")
+        return g
+
+    def train(self, X, T, steps=4000, batch=4096, lr=2e-3, graph=True, stream=None):
+        """Adam on both networks; returns the per-step loss curves [2][steps]."""
+        loss = np.zeros((2, steps), dtype=np.float32)
+        _check(load().upc_sur_train(self._s, _dptr(X), _dptr(T), X.shape[0], steps, batch, lr, int(graph),
+                                    _stream(stream), loss.ctypes.data_as(C.c_void_p)), "upc_sur_train")
+        return loss
+
+    def predict(self, X):
+        """Normalised predictions of both networks for any number of rows."""
+        torch = _dev()
+        n = X.shape[0]
+        n32 = (n + 31) // 32 * 32
+        if n32 != n:
+            X = torch.cat([X, torch.zeros((n32 - n, IN), dtype=X.dtype, device=X.device)])
+        outs = []
+        for net in (0, 1):
+            parts = [self.forward(net, X[o:o + 65536].contiguous()) for o in range(0, n32, 65536)]
+            outs.append(torch.cat(parts)[:n])
+        return outs
+
+    def log_lumi(self, X, logL0):
+        """(log L prediction, ensemble spread |r_a - r_b|) in float64; the
+        predicted error is err_scale times the spread."""
+        pa, pb = self.predict(X)
+        mu, sig = self.norm
+        ra, rb = mu + sig * pa.double(), mu + sig * pb.double()
+        return logL0 + 0.5 * (ra + rb), (ra - rb).abs()
+
+    def attach(self, ctx: Context, max_err=1e-3):
+        _check(load().upc_ctx_set_surrogate(ctx._ctx, self._s, max_err), "upc_ctx_set_surrogate")
+        ctx._surrogate = self  # keep alive while attached
+
+    def save(self, path):
+        mu, sig = self.norm
+        np.savez(path, net0=self.get_params(0), net1=self.get_params(1), mu=mu, sigma=sig,
+                 err_scale=self.err_scale)
+
+    @classmethod
+    def load(cls, path, seed=1):
+        d = np.load(path)
+        s = cls(seed)
+        s.set_params(0, d["net0"])
+        s.set_params(1, d["net1"])
+        s.norm = (float(d["mu"]), float(d["sigma"]))
+        s.err_scale = float(d["err_scale"])
+        return s
+
+
+def sample_systems(n, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        Z = int(rng.integers(8, 93))
+        A = int(round(2.5 * Z))
+        sqrts = float(np.exp(rng.uniform(np.log(200.0), np.log(40000.0))))
+        out.append(Config.make(Z=Z, A=A, R=wood_saxon_radius(A) * rng.uniform(0.95, 1.05),
+                               a=rng.uniform(0.42, 0.58), sqrts=sqrts, flux_point=0,
+                               breakup_mode=int(rng.integers(1, 5)), use_polarized=0, nm=2, ny=2))
+    return out
+
+
+def make_dataset(n_systems, per_system, seed, systems=None):
+    """Features, targets r = log L - log L0 and the labels, from the FP64 GPU
+    quadrature (upc_lumi_points) at random (M, Y) of random systems.
+    Points where L or L0 is zero or not finite are dropped."""
+    torch = _dev()
+    rng = np.random.default_rng(seed + 1)
+    systems = systems if systems is not None else sample_systems(n_systems, seed)
+    Xs, Rs, Ls, L0s = [], [], [], []
+    for cfg in systems:
+        M = np.exp(rng.uniform(0.0, np.log(200.0), per_system))
+        Y = rng.uniform(-7.0, 7.0, per_system)
+        ctx = Context(cfg)
+        L = ctx.lumi_points(torch.as_tensor(M, device="cuda"), torch.as_tensor(Y, device="cuda"))
+        ctx.close()
+        X, L0 = features(cfg, M, Y)
+        lg = torch.log(L)
+        ok = torch.isfinite(lg) & torch.isfinite(L0) & (L > 0)
+        Xs.append(X[ok])
+        Rs.append((lg - L0)[ok])
+        Ls.append(lg[ok])
+        L0s.append(L0[ok])
+    return torch.cat(Xs), torch.cat(Rs), torch.cat(Ls), torch.cat(L0s)
+
+
+def train_default(n_systems=256, per_system=4096, steps=6000, batch=4096, lr=2e-3, seed=11, n_valid=32):
+    """The default training run (n_systems x per_system points, 2^20 by
+    default) with the error scale calibrated on n_valid further systems.
+    Returns (surrogate, loss curves, training rows)."""
+    X, r, _, _ = make_dataset(n_systems, per_system, seed)
+    mu, sig = float(r.mean()), float(r.std())
+    T = ((r - mu) / sig).float().contiguous()
+    s = Surrogate(seed)
+    s.norm = (mu, sig)
+    loss = s.train(X.contiguous(), T, steps=steps, batch=batch, lr=lr)
+    Xv, _, lgv, L0v = make_dataset(0, 2048, seed + 5, systems=sample_systems(n_valid, seed + 7))
+    s.calibrate(Xv.contiguous(), L0v, lgv)
+    return s, loss, X.shape[0]
