@@ -137,6 +137,19 @@ int upc_nuclear_overlap(upc_ctx *ctx, const double *d_b, int n, int which, doubl
 int upc_lumi_points(upc_ctx *ctx, const double *d_M, const double *d_Y, int n, double *d_L,
                     double *d_Ls, double *d_Lp, void *stream);
 
+/* UpcCrossSection::calcPhotonFlux(M, Y): photon flux at k = M/2 e^Y weighted
+ * by P(min(b, 20)) G_AA(b) on the flux row's nodes (vector-meson path)
+ *                                              src/UpcCrossSection.cpp:699-720 */
+int upc_photon_flux(upc_ctx *ctx, const double *d_M, const double *d_Y, int n, double *d_out, void *stream);
+/* UpcCrossSection::calcNucCrossSectionY(cs, ratio, totCS) for a particle of
+ * mass mpart, with the elementary cross sections given at the ny lower bin
+ * edges: d_cs_pos[iy] = sigma(y), d_cs_neg[iy] = sigma(-y) (the reference
+ * takes them from UpcElemProcess::calcCrossSectionY).  d_cs[iy], d_ratio[iy];
+ * *h_tot += sum, then the sum is multiplied by dy as in the reference.
+ *                                              src/UpcCrossSection.cpp:722-748 */
+int upc_nuc_cs_y(upc_ctx *ctx, double mpart, const double *d_cs_pos, const double *d_cs_neg, double *d_cs,
+                 double *d_ratio, double *h_tot, void *stream);
+
 /* ---- grids ---- */
 /* prepareTwoPhotonLumi's grid: [nm][ny], lower bin edges, x dm x dy
  *                                              src/UpcCrossSection.cpp:494-576 */

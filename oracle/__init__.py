@@ -96,6 +96,9 @@ def lib():
         L.orc_nuc_cs_ym.argtypes = [vp, _DP, _DP, _DP, _DP, _DP, _DP]
         L.orc_photon_pt_hist.argtypes = [vp, d, d, _DP]
         L.orc_set_qags_tol.argtypes = [vp, d, d]
+        L.orc_photon_flux.argtypes = [vp, d, d]
+        L.orc_photon_flux.restype = d
+        L.orc_nuc_cs_y.argtypes = [vp, d, _DP, _DP, _DP, _DP, _DP]
         L.orc_pdf_init.restype = i
         L.orc_pdf_init.argtypes = [_DP, i, _DP]
         L.orc_pdf_sample.restype = d
@@ -173,6 +176,17 @@ class Oracle:
         out = np.zeros((self.params.nm, self.params.nz))
         lib().orc_fill_cs_zm(self._ctx, _ptr(out), flag)
         return out
+
+    def photon_flux(self, M, Y):
+        return lib().orc_photon_flux(self._ctx, M, Y)
+
+    def nuc_cs_y(self, mpart, cs_pos, cs_neg):
+        ny = self.params.ny
+        cs_pos = np.ascontiguousarray(cs_pos, dtype=np.float64)
+        cs_neg = np.ascontiguousarray(cs_neg, dtype=np.float64)
+        cs, ratio, tot = np.empty(ny), np.empty(ny), C.c_double(0.0)
+        lib().orc_nuc_cs_y(self._ctx, mpart, _ptr(cs_pos), _ptr(cs_neg), _ptr(cs), _ptr(ratio), C.byref(tot))
+        return cs, ratio, tot.value
 
     def set_qags_tol(self, epsabs, epsrel):
         lib().orc_set_qags_tol(self._ctx, epsabs, epsrel)

@@ -1146,6 +1146,52 @@ void orc_lumi_pol(const orc_ctx *c, double M, double Y, double *ns, double *np)
   lumi_pol_impl(c, M, Y, ns, np, &nbad);
 }
 
+/* calcPhotonFlux, src/UpcCrossSection.cpp:699-720: photon flux of one
+ * nucleus at k = M/2 e^Y, weighted by P(min(b, 20)) G(b) on the flux row's
+ * own nodes. */
+double orc_photon_flux(const orc_ctx *c, double M, double Y)
+{
+  const double R = c->p.R;
+  const double k = M / 2. * exp(Y);
+  const double bmin = c->p.flux_point ? 1 * R : 0.05 * R;
+  const double bmax = fmax(5. * c->g1 * kHc / k, 5 * R);
+  const double log_delta_b = (log(bmax) - log(bmin)) / NB1;
+  long nbad = 0;
+  double sum = 0;
+  for (int i = 0; i < NB1; i++) {
+    const double bl = bmin * exp(i * log_delta_b);
+    const double bh = bmin * exp((i + 1) * log_delta_b);
+    const double b = (bh + bl) / 2.;
+    double breakup = 1.;
+    if (c->p.breakup_mode != 1) breakup = orc_spline_eval(&c->breakp, b < 20. ? b : 20., NULL);
+    const double gaa = b < 20. ? orc_spline_eval(&c->gaa, b, NULL) : 1.;
+    sum += breakup * gaa * flux_form_checked(c, b, k, &nbad) * b * (bh - bl);
+  }
+  return 2. * M_PI * k * sum;
+}
+
+/* calcNucCrossSectionY, src/UpcCrossSection.cpp:722-748, with the
+ * elementary cross sections given: cs_pos[iy] = sigma(y), cs_neg[iy] =
+ * sigma(-y) at y = ymin + dy iy.  cs[iy] and ratio[iy]; *tot_cs += sum, then
+ * x dy. */
+void orc_nuc_cs_y(const orc_ctx *c, double mpart, const double *cs_pos, const double *cs_neg, double *cs,
+                  double *ratio, double *tot_cs)
+{
+  const int ny = c->p.ny;
+  const double dy = (c->p.ymax - c->p.ymin) / ny;
+  for (int iy = 0; iy < ny; iy++) {
+    const double y = c->p.ymin + dy * iy;
+    const double flux1 = orc_photon_flux(c, mpart, y);
+    const double flux2 = orc_photon_flux(c, mpart, -y);
+    const double u1 = flux1 * cs_pos[iy];
+    const double u2 = flux2 * cs_neg[iy];
+    cs[iy] = u1 + u2;
+    *tot_cs += u1 + u2;
+    ratio[iy] = u1 / u2;
+  }
+  *tot_cs *= dy;
+}
+
 /* src/UpcCrossSection.cpp:494-576: static contiguous M-row block per thread,
  * lower-edge evaluation, value x dm x dy. */
 long orc_lumi_grid(const orc_ctx *c, double *out, double *outS, double *outP)

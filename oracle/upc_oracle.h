@@ -80,6 +80,10 @@ void orc_lumi_pol(const orc_ctx *c, double M, double Y, double *ns, double *np);
  * out (unpolarised) or outS/outP (polarised).  Returns the number of QAGS
  * calls that ended with ier != 0 (the reference would abort on any). */
 long orc_lumi_grid(const orc_ctx *c, double *out, double *outS, double *outP);
+/* calcPhotonFlux(M, Y) and calcNucCrossSectionY (src/UpcCrossSection.cpp:699-748). */
+double orc_photon_flux(const orc_ctx *c, double M, double Y);
+void orc_nuc_cs_y(const orc_ctx *c, double mpart, const double *cs_pos, const double *cs_neg, double *cs,
+                  double *ratio, double *tot_cs);
 
 /* Elementary dilepton cross sections, src/UpcTwoPhotonDilep.cpp:46-134. */
 double orc_dilep_cs_m(int pdg, double lep_a, double m);

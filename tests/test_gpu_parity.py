@@ -230,3 +230,28 @@ def test_sampling_matches_tables(gpu_ctx):
     e = pm * n
     stat = np.sum((hm - e) ** 2 / e)
     assert stat < chi2.ppf(0.999, 39)
+
+
+@pytest.mark.parametrize("kw", [
+    dict(flux_point=0, breakup_mode=2),
+    dict(flux_point=1, breakup_mode=1),
+    dict(flux_point=0, breakup_mode=4),
+    dict(Z=54, A=129, R=5.4889, a=0.54, sqrts=5440.0, flux_point=0, breakup_mode=3),
+])
+def test_vm_photon_flux(gpu_ctx, oracle_ctx, kw):
+    """calcPhotonFlux (vector-meson photon flux) at J/psi and Upsilon masses."""
+    kw = dict(kw, nm=3, ny=9, ymin=-4.0, ymax=4.0)
+    g = gpu_ctx(**kw)
+    o = oracle_ctx(**kw)
+    M = np.array([3.0969] * 11 + [9.3987] * 11)
+    Y = np.tile(np.linspace(-5, 5, 11), 2)
+    got = _np(g.photon_flux(_t(M), _t(Y)))
+    ref = np.array([o.photon_flux(m, y) for m, y in zip(M, Y)])
+    assert np.all(ref > 0)
+    assert rel(got, ref) < 1e-9
+    rng = np.random.default_rng(8)
+    cs_pos, cs_neg = rng.uniform(0.5, 2, 9), rng.uniform(0.5, 2, 9)
+    cs, ratio, tot = g.nuc_cs_y(3.0969, _t(cs_pos), _t(cs_neg))
+    rcs, rratio, rtot = o.nuc_cs_y(3.0969, cs_pos, cs_neg)
+    assert rel(_np(cs), rcs) < 1e-9 and rel(_np(ratio), rratio) < 1e-9
+    assert tot == pytest.approx(rtot, rel=1e-9)

@@ -103,6 +103,8 @@ SIGNATURES = {
     "upc_nuclear_overlap": (_i, [_vp, _vp, _i, _i, _vp, _vp]),
     "upc_lumi_points": (_i, [_vp, _vp, _vp, _i, _vp, _vp, _vp, _vp]),
     "upc_lumi_grid": (_i, [_vp, _i, _vp, _vp, _vp, _vp]),
+    "upc_photon_flux": (_i, [_vp, _vp, _vp, _i, _vp, _vp]),
+    "upc_nuc_cs_y": (_i, [_vp, _d, _vp, _vp, _vp, _vp, C.POINTER(_d), _vp]),
     "upc_cs_zm": (_i, [_vp, _d, _d, _i, _d, _d, _i, _i, _vp, _vp]),
     "upc_cs_m": (_i, [_vp, _i, _vp, _vp]),
     "upc_elem_cs": (_i, [_vp, _i, _vp, _vp, _i, _vp, _vp]),
@@ -276,6 +278,23 @@ class Context:
         _check(load().upc_lumi_points(self._ctx, _dptr(M), _dptr(Y), n, _dptr(out), None, None,
                                       _stream(stream)), "upc_lumi_points")
         return out
+
+    def photon_flux(self, M, Y, stream=None):
+        """calcPhotonFlux(M, Y) at n points (vector-meson photon flux)."""
+        n = M.numel()
+        out = self._empty(n)
+        _check(load().upc_photon_flux(self._ctx, _dptr(M), _dptr(Y), n, _dptr(out), _stream(stream)),
+               "upc_photon_flux")
+        return out
+
+    def nuc_cs_y(self, mpart, cs_pos, cs_neg, stream=None):
+        """calcNucCrossSectionY with sigma(+-y) given: (cs[ny], ratio[ny], totCS)."""
+        ny = self.config.ny
+        cs, ratio = self._empty(ny), self._empty(ny)
+        tot = C.c_double(0.0)
+        _check(load().upc_nuc_cs_y(self._ctx, mpart, _dptr(cs_pos), _dptr(cs_neg), _dptr(cs), _dptr(ratio),
+                                   C.byref(tot), _stream(stream)), "upc_nuc_cs_y")
+        return cs, ratio, tot.value
 
     # --- grids ---
     def lumi_grid(self, form=FORM_FP64, out=None, stream=None):

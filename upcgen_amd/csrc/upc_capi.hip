@@ -261,6 +261,20 @@ int upc_lumi_grid(upc_ctx* c, int form, double* d_L, double* d_Ls, double* d_Lp,
   }
 }
 
+int upc_photon_flux(upc_ctx* c, const double* d_M, const double* d_Y, int n, double* d_out, void* stream)
+{
+  if (!c || n < 0 || (n > 0 && (!d_M || !d_Y || !d_out))) return set_error("bad arguments"), UPC_EINVAL;
+  return launch_photon_flux(c, d_M, d_Y, n, d_out, S(stream));
+}
+
+int upc_nuc_cs_y(upc_ctx* c, double mpart, const double* d_cs_pos, const double* d_cs_neg, double* d_cs,
+                 double* d_ratio, double* h_tot, void* stream)
+{
+  if (!c || !d_cs_pos || !d_cs_neg || !d_cs || !d_ratio || !h_tot || !(mpart > 0))
+    return set_error("bad arguments"), UPC_EINVAL;
+  return launch_nuc_cs_y(c, mpart, d_cs_pos, d_cs_neg, d_cs, d_ratio, h_tot, S(stream));
+}
+
 int upc_cs_zm(upc_ctx* c, double zmin, double zmax, int nz, double mmin, double mmax, int nm, int flag,
               double* d_zm, void* stream)
 {

@@ -253,6 +253,97 @@ int lumi_points_fp64(upc_ctx* c, const double* d_pts, int npts, double* d_L, dou
   return read_stats(c, s);
 }
 
+// ---- calcPhotonFlux / calcNucCrossSectionY (src/UpcCrossSection.cpp:699-748) ----
+// One wave per point: sum_j P(min(b,20)) G(b) N(b_j, k) b_j (b_h - b_l) over
+// the point's k1 flux row; flux = 2 pi k sum.
+__global__ __launch_bounds__(256) void k_vm_flux(int n, const double* __restrict__ row_k,
+                                                 const double* __restrict__ row_logd,
+                                                 const double* __restrict__ flux, FluxParams fp, int mode,
+                                                 const double4* __restrict__ gaa, const double4* __restrict__ bk,
+                                                 double* __restrict__ out)
+{
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int p = blockIdx.x * 4 + wv;
+  if (p >= n) return;
+  const double db = 20. / (kNB - 1);
+  double acc = 0;
+  for (int j = lane; j < kNB1; j += 64) {
+    double bl, bh;
+    const double b = row_node(fp.bmin, row_logd[p], j, &bl, &bh);
+    double breakup = 1.;
+    if (mode != 1) breakup = spline_eval(bk, kBkMin, kDBk, 1.0 / kDBk, kNBkUsed, b < 20. ? b : 20.);
+    const double g = b < 20. ? spline_eval(gaa, 0.0, db, 1.0 / db, kNB, b) : 1.;
+    acc += breakup * g * flux[(long)p * kNB1 + j] * b * (bh - bl);
+  }
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+  if (lane == 0) out[p] = 2. * kPi * row_k[p] * acc;
+}
+
+int launch_photon_flux(upc_ctx* c, const double* d_M, const double* d_Y, int n, double* d_out, hipStream_t s)
+{
+  if (n <= 0) return 0;
+  double* pts = nullptr;
+  UPC_HIP(hipMallocAsync((void**)&pts, sizeof(double) * 2 * n, s));
+  UPC_HIP(hipMemcpyAsync(pts, d_M, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+  UPC_HIP(hipMemcpyAsync(pts + n, d_Y, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+  FluxParams fp;
+  int rc = prepare_flux(c, pts, n, &fp, s);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_vm_flux, dim3(nblk(n, 4)), dim3(256), 0, s, n, c->work.d_row_k, c->work.d_row_logd,
+                     c->work.d_flux, fp, c->cfg.breakup_mode, c->d_gaa, c->d_bk, d_out);
+  UPC_HIP(hipGetLastError());
+  (void)hipFreeAsync(pts, s);
+  return read_stats(c, s);
+}
+
+// cs[iy] = F(y) s(y) + F(-y) s(-y), ratio = first / second, total summed in
+// iy order then x dy (single thread: the reference's order).
+__global__ void k_nuc_cs_y(int ny, double dy, const double* __restrict__ f, const double* __restrict__ cs_pos,
+                           const double* __restrict__ cs_neg, double* __restrict__ cs, double* __restrict__ ratio,
+                           double* __restrict__ tot)
+{
+  double t = 0;
+  for (int iy = 0; iy < ny; ++iy) {
+    const double u1 = f[iy] * cs_pos[iy];
+    const double u2 = f[ny + iy] * cs_neg[iy];
+    cs[iy] = u1 + u2;
+    t += u1 + u2;
+    ratio[iy] = u1 / u2;
+  }
+  *tot = t * dy;
+}
+
+__global__ void k_vm_points(int ny, double ymin, double dy, double mpart, double* __restrict__ MY)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ny) return;
+  const double y = ymin + dy * i;
+  MY[i] = mpart;
+  MY[ny + i] = mpart;
+  MY[2 * ny + i] = y;
+  MY[3 * ny + i] = -y;
+}
+
+int launch_nuc_cs_y(upc_ctx* c, double mpart, const double* d_cs_pos, const double* d_cs_neg, double* d_cs,
+                    double* d_ratio, double* h_tot, hipStream_t s)
+{
+  const int ny = c->cfg.ny;
+  const double dy = (c->cfg.ymax - c->cfg.ymin) / ny;
+  double* buf = nullptr;  // M[2 ny], Y[2 ny], flux[2 ny], total
+  UPC_HIP(hipMallocAsync((void**)&buf, sizeof(double) * (6 * ny + 1), s));
+  hipLaunchKernelGGL(k_vm_points, dim3(nblk(ny, 256)), dim3(256), 0, s, ny, c->cfg.ymin, dy, mpart, buf);
+  int rc = launch_photon_flux(c, buf, buf + 2 * ny, 2 * ny, buf + 4 * ny, s);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_nuc_cs_y, dim3(1), dim3(1), 0, s, ny, dy, buf + 4 * ny, d_cs_pos, d_cs_neg, d_cs, d_ratio,
+                     buf + 6 * ny);
+  double t = 0;
+  UPC_HIP(hipMemcpyAsync(&t, buf + 6 * ny, sizeof(double), hipMemcpyDeviceToHost, s));
+  UPC_HIP(hipStreamSynchronize(s));
+  *h_tot += t;
+  (void)hipFreeAsync(buf, s);
+  return 0;
+}
+
 // ---- batched per-point flux (C-ABI upc_flux_point / upc_flux_form) ----
 __global__ void k_flux_point_pts(const double* __restrict__ b, const double* __restrict__ k, int n,
                                  FluxParams fp, double* __restrict__ out)

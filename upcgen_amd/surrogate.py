@@ -97,11 +97,23 @@ class Surrogate:
     def err_scale(self, v):
         _check(load().upc_sur_set_err_scale(self._s, float(v)), "upc_sur_set_err_scale")
 
-    def calibrate(self, X, logL0, log_true, quantile=0.99):
-        """Error scale from held-out points: the given quantile of |error| / spread."""
+    def calibrate(self, X, logL0, log_true, max_err=1e-3, coverage=0.99):
+        """Error scale from held-out points: the smallest scale c such that,
+        of the points the gate would serve (c * spread <= max_err), at least
+        `coverage` are within max_err.  When no scale achieves that, c is set
+        so large that nothing is served."""
         pred, spread = self.log_lumi(X, logL0)
-        ratio = ((pred - log_true).abs() / spread.clamp_min(1e-12)).cpu().numpy()
-        self.err_scale = float(max(1.0, np.quantile(ratio, quantile)))
+        err = (pred - log_true).abs().cpu().numpy()
+        spr = np.maximum(spread.cpu().numpy(), 1e-300)
+        order = np.argsort(spr)
+        ok = np.cumsum(err[order] <= max_err) / np.arange(1, len(order) + 1)
+        # serving the k smallest spreads is acceptable when the running coverage holds
+        good = np.nonzero(ok >= coverage)[0]
+        if good.size == 0:
+            self.err_scale = 1e300
+            return self.err_scale
+        k = good.max()
+        self.err_scale = float(max(1.0, max_err / spr[order[k]]))
         return self.err_scale
 
     def forward(self, net: int, X, stream=None):
