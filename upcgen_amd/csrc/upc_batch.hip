@@ -75,66 +75,6 @@ __global__ void k_batch_rows(const BatchJob* __restrict__ jobs, const int* __res
   row_setup(J.v.pts, J.v.npts, (int)(r - J.row0), J.v.fp, &row_k[r], &row_logd[r], &row_cnt[r]);
 }
 
-// Three-pass exclusive scan of the row counts: per-block sums, their scan in
-// one block, then the offsets.  *total receives the sum.
-constexpr int kScanBlock = 1024;
-
-__global__ __launch_bounds__(kScanBlock) void k_scan_part(const int* __restrict__ cnt, long n,
-                                                          long long* __restrict__ part)
-{
-  __shared__ long long s[kScanBlock];
-  const long i = (long)blockIdx.x * kScanBlock + threadIdx.x;
-  s[threadIdx.x] = i < n ? cnt[i] : 0;
-  __syncthreads();
-  for (int d = kScanBlock / 2; d > 0; d >>= 1) {
-    if (threadIdx.x < d) s[threadIdx.x] += s[threadIdx.x + d];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) part[blockIdx.x] = s[0];
-}
-
-__global__ __launch_bounds__(kScanBlock) void k_scan_top(long long* __restrict__ part, int nparts,
-                                                         unsigned long long* __restrict__ total)
-{
-  __shared__ long long s[kScanBlock];
-  long long carry = 0;
-  for (int base = 0; base < nparts; base += kScanBlock) {
-    const int i = base + threadIdx.x;
-    const long long v = i < nparts ? part[i] : 0;
-    s[threadIdx.x] = v;
-    __syncthreads();
-    for (int d = 1; d < kScanBlock; d <<= 1) {
-      const long long t = threadIdx.x >= d ? s[threadIdx.x - d] : 0;
-      __syncthreads();
-      s[threadIdx.x] += t;
-      __syncthreads();
-    }
-    if (i < nparts) part[i] = carry + s[threadIdx.x] - v;  // exclusive
-    const long long blk_total = s[kScanBlock - 1];
-    __syncthreads();
-    carry += blk_total;
-  }
-  if (threadIdx.x == 0) *total = (unsigned long long)carry;
-}
-
-__global__ __launch_bounds__(kScanBlock) void k_scan_apply(const int* __restrict__ cnt, long n,
-                                                           const long long* __restrict__ part,
-                                                           int* __restrict__ off)
-{
-  __shared__ long long s[kScanBlock];
-  const long i = (long)blockIdx.x * kScanBlock + threadIdx.x;
-  const long long v = i < n ? cnt[i] : 0;
-  s[threadIdx.x] = v;
-  __syncthreads();
-  for (int d = 1; d < kScanBlock; d <<= 1) {
-    const long long t = threadIdx.x >= d ? s[threadIdx.x - d] : 0;
-    __syncthreads();
-    s[threadIdx.x] += t;
-    __syncthreads();
-  }
-  if (i < n) off[i] = (int)(part[blockIdx.x] + s[threadIdx.x] - v);
-}
-
 __global__ void k_batch_point_rows(const BatchJob* __restrict__ jobs, const int* __restrict__ row0, int njobs,
                                    long total_rows, const double* __restrict__ row_k,
                                    const double* __restrict__ row_logd, const int* __restrict__ row_cnt,
@@ -255,9 +195,9 @@ int enqueue(upc_batch* b, hipStream_t s)
                      b->npts, b->d_pts);
   hipLaunchKernelGGL(k_batch_rows, dim3(nblk(b->nrows, 256)), dim3(256), 0, s, b->d_jobs, b->d_row0, nj,
                      b->nrows, b->d_row_k, b->d_row_logd, b->d_row_cnt);
-  hipLaunchKernelGGL(k_scan_part, dim3(b->nparts), dim3(kScanBlock), 0, s, b->d_row_cnt, b->nrows, b->d_part);
-  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanBlock), 0, s, b->d_part, b->nparts, b->d_counters + 4);
-  hipLaunchKernelGGL(k_scan_apply, dim3(b->nparts), dim3(kScanBlock), 0, s, b->d_row_cnt, b->nrows, b->d_part,
+  hipLaunchKernelGGL(k_scan_part<>, dim3(b->nparts), dim3(kScanBlock), 0, s, b->d_row_cnt, b->nrows, b->d_part);
+  hipLaunchKernelGGL(k_scan_top<>, dim3(1), dim3(kScanBlock), 0, s, b->d_part, b->nparts, b->d_counters + 4);
+  hipLaunchKernelGGL(k_scan_apply<>, dim3(b->nparts), dim3(kScanBlock), 0, s, b->d_row_cnt, b->nrows, b->d_part,
                      b->d_row_off);
   hipLaunchKernelGGL(k_batch_point_rows, dim3(nblk(b->nrows * kNB1, 256)), dim3(256), 0, s, b->d_jobs, b->d_row0,
                      nj, b->nrows, b->d_row_k, b->d_row_logd, b->d_row_cnt, b->d_row_off, b->d_flux, b->d_tasks);

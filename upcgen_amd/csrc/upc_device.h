@@ -53,14 +53,15 @@ constexpr int kNB1 = 120, kNB2 = 120, kNB = 200;
 // Chebyshev / polynomial evaluation
 // ---------------------------------------------------------------------------
 // Coefficients live in __constant__ memory and are read with uniform (scalar)
-// loads inside rolled loops: unrolled loops of FP64 immediates get hoisted
-// into ~2 VGPRs per coefficient by the loop optimiser.
+// loads inside partly unrolled loops (4 terms per trip: one scalar-load wait
+// per trip); fully unrolled loops of FP64 immediates get hoisted into ~2
+// VGPRs per coefficient by the loop optimiser.
 template <int N>
 __device__ __forceinline__ double clenshaw(const double (&c)[N], double u)
 {
   double b1 = 0.0, b2 = 0.0;
   const double u2 = 2.0 * u;
-#pragma unroll 1
+#pragma unroll 4
   for (int k = N - 1; k >= 1; --k) {
     const double t = c[k] + u2 * b1 - b2;
     b2 = b1;
@@ -76,7 +77,7 @@ __device__ __forceinline__ void clenshaw2(const double (&c)[N], const double (&d
 {
   double b1 = 0.0, b2 = 0.0, e1 = 0.0, e2 = 0.0;
   const double u2 = 2.0 * u;
-#pragma unroll 1
+#pragma unroll 4
   for (int k = N - 1; k >= 1; --k) {
     const double t = c[k] + u2 * b1 - b2;
     b2 = b1;
@@ -139,12 +140,17 @@ __device__ __forceinline__ void sincos_medium(double x, double* s, double* c)
 }
 
 // J1(x), x >= 0 (role of gsl_sf_bessel_J1 in src/UpcCrossSection.cpp:189).
-__device__ __forceinline__ double bessel_j1(double x)
+// J1 on its two branches (x <= 8: Chebyshev series in (x/4)^2; x > 8:
+// modulus/phase form), callable separately so a caller can evaluate the
+// lanes of one branch together.
+__device__ __forceinline__ double bessel_j1_small(double x)
 {
-  if (x <= 8.0) {
-    const double u = x * x * (1.0 / 32.0) - 1.0;
-    return x * clenshaw(kJ1Small, u);
-  }
+  const double u = x * x * (1.0 / 32.0) - 1.0;
+  return x * clenshaw(kJ1Small, u);
+}
+
+__device__ __forceinline__ double bessel_j1_large(double x)
+{
   const double r = 1.0 / x;
   const double v = 128.0 * r * r - 1.0;
   double P, Q;
@@ -153,6 +159,11 @@ __device__ __forceinline__ double bessel_j1(double x)
   double s, c;
   sincos_medium(x, &s, &c);
   return (P * (s - c) + Q * (s + c)) / sqrt(kPi * x);
+}
+
+__device__ __forceinline__ double bessel_j1(double x)
+{
+  return x <= 8.0 ? bessel_j1_small(x) : bessel_j1_large(x);
 }
 
 // K0(x), K1(x) for x > 0 (role of gsl_sf_bessel_K0/K1 in

@@ -488,6 +488,69 @@ __global__ __launch_bounds__(64 * WAVES) void k_lumi_fp64(JR jr)
   else lumi_fp64_body<false, WAVES>(v, p0, sm);
 }
 
+// Three-pass exclusive scan of the row counts: per-block sums, their scan in
+// one block, then the offsets.  *total receives the sum.
+constexpr int kScanBlock = 1024;
+
+template <int BS = kScanBlock>
+__global__ __launch_bounds__(BS) void k_scan_part(const int* __restrict__ cnt, long n,
+                                                          long long* __restrict__ part)
+{
+  __shared__ long long s[BS];
+  const long i = (long)blockIdx.x * BS + threadIdx.x;
+  s[threadIdx.x] = i < n ? cnt[i] : 0;
+  __syncthreads();
+  for (int d = BS / 2; d > 0; d >>= 1) {
+    if (threadIdx.x < d) s[threadIdx.x] += s[threadIdx.x + d];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
+template <int BS = kScanBlock>
+__global__ __launch_bounds__(BS) void k_scan_top(long long* __restrict__ part, int nparts,
+                                                         unsigned long long* __restrict__ total)
+{
+  __shared__ long long s[BS];
+  long long carry = 0;
+  for (int base = 0; base < nparts; base += BS) {
+    const int i = base + threadIdx.x;
+    const long long v = i < nparts ? part[i] : 0;
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int d = 1; d < BS; d <<= 1) {
+      const long long t = threadIdx.x >= d ? s[threadIdx.x - d] : 0;
+      __syncthreads();
+      s[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (i < nparts) part[i] = carry + s[threadIdx.x] - v;  // exclusive
+    const long long blk_total = s[BS - 1];
+    __syncthreads();
+    carry += blk_total;
+  }
+  if (threadIdx.x == 0) *total = (unsigned long long)carry;
+}
+
+template <int BS = kScanBlock>
+__global__ __launch_bounds__(BS) void k_scan_apply(const int* __restrict__ cnt, long n,
+                                                           const long long* __restrict__ part,
+                                                           int* __restrict__ off)
+{
+  __shared__ long long s[BS];
+  const long i = (long)blockIdx.x * BS + threadIdx.x;
+  const long long v = i < n ? cnt[i] : 0;
+  s[threadIdx.x] = v;
+  __syncthreads();
+  for (int d = 1; d < BS; d <<= 1) {
+    const long long t = threadIdx.x >= d ? s[threadIdx.x - d] : 0;
+    __syncthreads();
+    s[threadIdx.x] += t;
+    __syncthreads();
+  }
+  if (i < n) off[i] = (int)(part[blockIdx.x] + s[threadIdx.x] - v);
+}
+
 inline FluxParams flux_params(const upc_ctx* c, int mirror_ny = 0)
 {
   FluxParams fp;

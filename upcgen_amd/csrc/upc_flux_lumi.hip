@@ -46,32 +46,6 @@ __global__ void k_rows(const double* __restrict__ pts, int npts, int nrows, Flux
   row_setup(pts, npts, r, fp, &row_k[r], &row_logd[r], &row_cnt[r]);
 }
 
-// Exclusive scan of the row counts (one block); writes the total to *total.
-__global__ __launch_bounds__(1024) void k_scan_counts(const int* __restrict__ cnt, int n,
-                                                      int* __restrict__ off, unsigned long long* total)
-{
-  __shared__ long long part[1024];
-  const int t = threadIdx.x;
-  const int per = (n + 1023) / 1024;
-  const int s = t * per, e = min(n, s + per);
-  long long acc = 0;
-  for (int i = s; i < e; ++i) acc += cnt[i];
-  part[t] = acc;
-  __syncthreads();
-  for (int d = 1; d < 1024; d <<= 1) {
-    const long long v = t >= d ? part[t - d] : 0;
-    __syncthreads();
-    part[t] += v;
-    __syncthreads();
-  }
-  long long base = t ? part[t - 1] : 0;
-  for (int i = s; i < e; ++i) {
-    off[i] = (int)base;
-    base += cnt[i];
-  }
-  if (t == 1023) *total = (unsigned long long)part[1023];
-}
-
 // Point-charge flux for nodes with b > 2R (or FLUX_POINT 1); QAGS task ids
 // for the others.
 __global__ void k_flux_point_rows(int nrows, FluxParams fp, const double* __restrict__ row_k,
@@ -113,6 +87,7 @@ static int ensure_work(upc_ctx* c, size_t npts)
   UPC_HIP(hipMalloc(&w.d_overflow, sizeof(int) * rows * kNB1));
   UPC_HIP(hipMalloc(&w.d_counters, sizeof(unsigned long long) * 8));
   UPC_HIP(hipMalloc(&w.d_points, sizeof(double) * 4 * npts));
+  UPC_HIP(hipMalloc(&w.d_part, sizeof(long long) * ((rows + 1023) / 1024 + 1)));
   w.rows_cap = rows;
   w.tasks_cap = rows * kNB1;
   w.points_cap = npts;
@@ -123,7 +98,7 @@ void free_work(upc_ctx* c)
 {
   LumiWork& w = c->work;
   void* ptrs[] = {w.d_row_k, w.d_row_logd, w.d_row_cnt, w.d_row_off, w.d_flux, w.d_tasks,
-                  w.d_overflow, w.d_counters, w.d_points};
+                  w.d_overflow, w.d_counters, w.d_points, w.d_part};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   w = LumiWork{};
@@ -161,8 +136,11 @@ static int run_flux(upc_ctx* c, int npts, int mirror_ny, hipStream_t s)
   UPC_HIP(hipMemsetAsync(w.d_counters, 0, sizeof(unsigned long long) * 8, s));
   hipLaunchKernelGGL(k_rows, dim3(nblk(nrows, 256)), dim3(256), 0, s, w.d_points, npts, nrows, fp, w.d_row_k,
                      w.d_row_logd, w.d_row_cnt);
-  hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, s, w.d_row_cnt, nrows, w.d_row_off,
-                     w.d_counters + 4);
+  const int nparts = nblk(nrows, kScanBlock);
+  hipLaunchKernelGGL(k_scan_part<>, dim3(nparts), dim3(kScanBlock), 0, s, w.d_row_cnt, (long)nrows, w.d_part);
+  hipLaunchKernelGGL(k_scan_top<>, dim3(1), dim3(kScanBlock), 0, s, w.d_part, nparts, w.d_counters + 4);
+  hipLaunchKernelGGL(k_scan_apply<>, dim3(nparts), dim3(kScanBlock), 0, s, w.d_row_cnt, (long)nrows, w.d_part,
+                     w.d_row_off);
   hipLaunchKernelGGL(k_flux_point_rows, dim3(nblk((long)nrows * kNB1, 256)), dim3(256), 0, s, nrows, fp,
                      w.d_row_k, w.d_row_logd, w.d_row_cnt, w.d_row_off, w.d_flux, w.d_tasks);
   UPC_HIP(hipGetLastError());

@@ -41,7 +41,8 @@ struct LumiWork {
   int* d_tasks = nullptr;         // row*128 + j
   unsigned long long* d_counters = nullptr;  // [0] task cursor, [1] qags bad, [2] evals, [3] overflow n
   int* d_overflow = nullptr;      // task ids that exceeded the fast workspace
-  double* d_points = nullptr;     // (M, Y, scale) per bin for point lists
+  double* d_points = nullptr;     // (M, Y, dm, dy) per point
+  long long* d_part = nullptr;    // per-block partial sums of the row-count scan
   size_t points_cap = 0;
 };
 
