@@ -210,8 +210,13 @@ struct PointTask {  // arbitrary (b, k) points of one context
 // wavefront keeps issuing full-width QK21 passes.  stats: [0] integrand
 // evaluations, [1] integrals with ier != 0, [2] overflow count.
 // ---------------------------------------------------------------------------
+// Waves per SIMD the QAGS kernel is compiled for (its register budget).
+#ifndef UPC_QAGS_MIN_WAVES
+#define UPC_QAGS_MIN_WAVES 4
+#endif
+
 template <int CAP, class TS>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_flux_qags(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(UPC_QAGS_MIN_WAVES, 8))) void k_flux_qags(
     TS ts, const unsigned long long* __restrict__ ntasks_p, unsigned long long* __restrict__ cursor,
     double* __restrict__ flux, int* __restrict__ ier_out, unsigned long long* __restrict__ stats,
     int* __restrict__ overflow)

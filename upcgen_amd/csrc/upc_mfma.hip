@@ -64,7 +64,8 @@ __device__ __forceinline__ double edge_e(double b, double J, double s)
   return e;
 }
 
-// J and -g'(20-) of g(b) = P(b) G(b) (P = 1 in mode 1) from the splines.
+// J and -g'(20-) of g(b) = P(b) G(b) (P = 1 in mode 1) from the splines,
+// and the five phi-node cosines (edge[2..6]).
 __global__ void k_mfma_edge_params(int mode, const double4* __restrict__ gaa, const double4* __restrict__ bk,
                                    const DevScalars* __restrict__ sc, double* __restrict__ edge)
 {
@@ -78,6 +79,8 @@ __global__ void k_mfma_edge_params(int mode, const double4* __restrict__ gaa, co
   }
   edge[0] = P - P * G;  // g(20) = P(20) exactly; g(20-) = P(20) G(20)
   edge[1] = -(dP * G + P * dG);
+#pragma unroll
+  for (int k = 0; k < 5; ++k) edge[2 + k] = cos(kPi * kGLx5[k]);  // the phi-node cosines, once
 }
 
 struct MfmaGrid {
@@ -105,7 +108,7 @@ struct MfmaState {
   double* d_wsum = nullptr;   // [rows] sum of all 120 W_i
   int* d_ulast = nullptr;     // [rows] last nonzero What column
   double* d_edge_sum = nullptr;  // [npol][points] exact edge part
-  double* d_edge = nullptr;      // J, -g'(20-)
+  double* d_edge = nullptr;      // J, -g'(20-), the five phi-node cosines
   unsigned long long* d_ksteps = nullptr;
   unsigned long long last_ksteps = 0;
 };
@@ -309,7 +312,7 @@ __global__ __launch_bounds__(256) void k_mfma_edge(int npts, const double* __res
   constexpr double kCore2 = (20. - kEdgeWidth) * (20. - kEdgeWidth);
   double cosk[5];
 #pragma unroll
-  for (int k = 0; k < 5; ++k) cosk[k] = cos(kPi * kGLx5[k]);
+  for (int k = 0; k < 5; ++k) cosk[k] = edge[2 + k];
   double acc_s = 0, acc_p = 0;
   for (int t = lane; t < kNB1 * 5; t += 64) {
     const int i = t / 5, k = t - 5 * (t / 5);
@@ -318,6 +321,8 @@ __global__ __launch_bounds__(256) void k_mfma_edge(int npts, const double* __res
     for (int q = 1; q < 5; ++q) cs = k == q ? cosk[q] : cs;
     const double c = POL ? -cs : cs;
     const double b1 = sb1[wv][i];
+    // no node pair reaches b < 20 when even the closest approach b1 sin(phi) does not
+    if (b1 * b1 * (1.0 - c * c) >= 400.0 * (1.0 + 1e-12)) continue;
     auto bk2 = [&](int j) { return node_b2(b1, sb2[wv][j], c); };
     // sqrt(x) < 20 exactly as the node loop tests it: x >= 400 is never
     // inside, and only x within rounding of 400 needs the square root
@@ -564,7 +569,7 @@ static int ensure_buffers(MfmaState* st, size_t nrows)
     UPC_HIP(hipMalloc(&st->d_Dl, dn * 2));
     UPC_HIP(hipMalloc(&st->d_D32, dn * 4));
     UPC_HIP(hipMalloc(&st->d_ksteps, sizeof(unsigned long long)));
-    UPC_HIP(hipMalloc(&st->d_edge, 2 * sizeof(double)));
+    UPC_HIP(hipMalloc(&st->d_edge, 8 * sizeof(double)));
     st->d_cap = dn;
   }
   const size_t wn = nrows * g.nu_pad;
