@@ -40,6 +40,10 @@ KSTEP_FLOP = 2 * 16 * 16 * 32
 #   sin/cos with argument reduction 40, combination + sqrt + divide 8,
 #   product 1, QK21 Gauss/Kronrod/abs sums 8
 FLOP_PER_QAGS_EVAL = 153
+# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (one
+# counter per pass) on this workload, scaled by a 1 GiB copy's readings; the
+# commands and raw counts are in profiles/pmc_traffic_1001x121_xnxn.txt.
+TRAFFIC_BYTES = {"k_flux_qags": 1.724e11, "k_lumi_fp64": 2.357e8, "k_mfma_lumi_split": 5.527e9}
 HBM_PEAK_GBS = 8000.0
 
 
@@ -250,7 +254,8 @@ def main():
     qags_tf = qags_flop / (qags_ms * 1e-3) / 1e12 if qags_ms > 0 else 0.0
     roof_qags = {"bound": "mfma", "kernel": "k_flux_qags (QAGS form-factor flux, FP64 VALU)",
                  "achieved": round(qags_tf, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                 "frac": round(qags_tf / FP64_PEAK_TFLOPS, 5), "traffic": None,
+                 "frac": round(qags_tf / FP64_PEAK_TFLOPS, 5), "traffic": TRAFFIC_BYTES["k_flux_qags"],
+                 "traffic_unit": "bytes per launch (rocprofv3 --pmc, copy-calibrated)",
                  "algorithmic_flop_per_launch": qags_flop, "integrand_evals_per_launch": inf.qags_evals,
                  "flop_per_eval": FLOP_PER_QAGS_EVAL, "avg_launch_ms": round(qags_ms, 4)}
     if form == U.FORM_FP64:
@@ -259,7 +264,7 @@ def main():
         tflops = flop / (lumi_ms * 1e-3) / 1e12 if lumi_ms > 0 else 0.0
         roof = {"bound": "mfma", "kernel": "k_lumi_fp64 (luminosity contraction, FP64 VALU)",
                 "achieved": round(tflops, 4), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(tflops / FP64_PEAK_TFLOPS, 5), "traffic": None,
+                "frac": round(tflops / FP64_PEAK_TFLOPS, 5), "traffic": TRAFFIC_BYTES["k_lumi_fp64"],
                 "algorithmic_flop_per_launch": flop, "avg_launch_ms": round(lumi_ms, 4)}
     else:
         # MFMA FLOP issued per launch: tile steps x 16x16x32 x products per step
@@ -271,7 +276,8 @@ def main():
         tflops = flop / (lumi_ms * 1e-3) / 1e12 if lumi_ms > 0 else 0.0
         roof = {"bound": "mfma", "kernel": f"k_mfma_lumi ({'split-bf16' if split else 'fp32'} banded GEMM)",
                 "achieved": round(tflops, 3), "peak": peak, "unit": "TFLOP/s",
-                "frac": round(tflops / peak, 5), "traffic": None, "mfma_flop_per_launch": flop,
+                "frac": round(tflops / peak, 5),
+                "traffic": TRAFFIC_BYTES["k_mfma_lumi_split"] if split else None, "mfma_flop_per_launch": flop,
                 "ksteps_per_launch": inf.mfma_ksteps, "avg_launch_ms": round(lumi_ms, 4),
                 "shared_grid_spacing_fm": inf.mfma_spacing}
     compare = None
