@@ -184,3 +184,18 @@ def test_pair_kinematics(gpu_ctx):
     eta = np.arcsinh(p[..., 2] / pt)
     passed = (pt >= 1.0).all(axis=1) & (np.abs(eta) <= 1.5).all(axis=1)
     assert np.array_equal(passed, ok == 1)
+
+
+def test_cli_mfma_form(tmp_path):
+    """-form mfma: the same run with the MFMA luminosity; total cross section
+    within the MFMA form's 1e-5 of the FP64 run."""
+    write_params(tmp_path / "p.in", NEVENTS=10, USE_HEPMC_OUTPUT=0, FLUX_POINT=0, BREAKUP_MODE=2)
+    rc, out = run(tmp_path)
+    assert rc == 0, out[-2000:]
+    tot64 = logged(out, "totCS_mb")
+    # the cache is keyed by the form: the MFMA run does not reuse the FP64 grid
+    rc, out = run(tmp_path, "-form", "mfma")
+    assert rc == 0, out[-2000:]
+    assert "different parameters" in out and "Starting all over" in out
+    tot = logged(out, "totCS_mb")
+    assert tot == pytest.approx(tot64, rel=1e-5) and tot != tot64

@@ -43,13 +43,15 @@ struct LumiCacheHeader {
   char magic[8];
   int32_t version, polarized, Z, A, flux_point, breakup_mode, nm, ny;
   double R, a, sqrts, mmin, mmax, ymin, ymax;
+  int32_t form, pad;  // luminosity form that made the grid (UPC_FORM_*)
 };
 
-LumiCacheHeader makeHeader(const upc_config& c)
+LumiCacheHeader makeHeader(const upc_config& c, int form)
 {
   LumiCacheHeader h{};
-  std::memcpy(h.magic, "UPCLUMI1", 8);
-  h.version = 1;
+  std::memcpy(h.magic, "UPCLUMI2", 8);
+  h.version = 2;
+  h.form = form;
   h.polarized = c.use_polarized;
   h.Z = c.Z;
   h.A = c.A;
@@ -71,7 +73,8 @@ bool sameKey(const LumiCacheHeader& x, const LumiCacheHeader& y)
 {
   return x.polarized == y.polarized && x.Z == y.Z && x.A == y.A && x.flux_point == y.flux_point &&
          x.breakup_mode == y.breakup_mode && x.nm == y.nm && x.ny == y.ny && x.R == y.R && x.a == y.a &&
-         x.sqrts == y.sqrts && x.mmin == y.mmin && x.mmax == y.mmax && x.ymin == y.ymin && x.ymax == y.ymax;
+         x.sqrts == y.sqrts && x.mmin == y.mmin && x.mmax == y.mmax && x.ymin == y.ymin && x.ymax == y.ymax &&
+         x.form == y.form;
 }
 }  // namespace
 
@@ -327,11 +330,11 @@ bool UpcCrossSection::readLumiCache(std::vector<double>& l0, std::vector<double>
   if (!f) return false;
   LumiCacheHeader h{};
   f.read(reinterpret_cast<char*>(&h), sizeof(h));
-  if (!f || std::memcmp(h.magic, "UPCLUMI1", 8) != 0) {
+  if (!f || std::memcmp(h.magic, "UPCLUMI2", 8) != 0) {
     UPC_LOG_WARNING << "Cached luminosity file " << lumiFileName() << " is not readable: recomputing";
     return false;
   }
-  const LumiCacheHeader want = makeHeader(config());
+  const LumiCacheHeader want = makeHeader(config(), lumiForm);
   if (!sameKey(h, want)) {
     if (requireKey) {
       UPC_LOG_WARNING << "Cached luminosity " << lumiFileName()
@@ -356,7 +359,7 @@ void UpcCrossSection::writeLumiCache(const std::vector<double>& l0, const std::v
   const std::string tmp = lumiFileName() + ".tmp";
   {
     std::ofstream f(tmp, std::ios::binary | std::ios::trunc);
-    const LumiCacheHeader h = makeHeader(config());
+    const LumiCacheHeader h = makeHeader(config(), lumiForm);
     f.write(reinterpret_cast<const char*>(&h), sizeof(h));
     f.write(reinterpret_cast<const char*>(l0.data()), sizeof(double) * l0.size());
     if (usePolarizedCS) f.write(reinterpret_cast<const char*>(l1.data()), sizeof(double) * l1.size());
