@@ -109,6 +109,12 @@ int upc_ctx_set_timing(upc_ctx *ctx, int enable);
  * accepted range [0.05, 2]).  No reference counterpart: the reference has
  * only the per-bin node quadrature. */
 int upc_ctx_set_mfma_spacing(upc_ctx *ctx, double h_fm);
+/* Intervals (1..16, default 16) a form-factor flux integral may hold in the
+ * fast QAGS kernel before it is handed to the kernel with the reference's
+ * 1000-interval workspace.  The result does not depend on it (both kernels
+ * run the same arithmetic); smaller values exercise the second kernel.  No
+ * reference counterpart. */
+int upc_ctx_set_qags_cap(upc_ctx *ctx, int cap);
 int upc_ctx_timings(upc_ctx *ctx, upc_timings *h_out, int reset);
 
 /* ---- per-point functions of the path (batched; device arrays of n) ---- */

@@ -96,6 +96,7 @@ SIGNATURES = {
     "upc_ctx_set_timing": (_i, [_vp, _i]),
     "upc_ctx_timings": (_i, [_vp, C.POINTER(Timings), _i]),
     "upc_ctx_set_mfma_spacing": (_i, [_vp, _d]),
+    "upc_ctx_set_qags_cap": (_i, [_vp, _i]),
     "upc_flux_point": (_i, [_vp, _vp, _vp, _i, _vp, _vp]),
     "upc_flux_form": (_i, [_vp, _vp, _vp, _i, _vp, _vp, _vp]),
     "upc_form_factor": (_i, [_vp, _vp, _i, _i, _vp, _vp]),
@@ -216,6 +217,11 @@ class Context:
     def set_mfma_spacing(self, h_fm):
         """Node spacing [fm] of the MFMA forms' shared b grid."""
         _check(load().upc_ctx_set_mfma_spacing(self._ctx, h_fm), "upc_ctx_set_mfma_spacing")
+
+    def set_qags_cap(self, cap):
+        """Intervals the fast QAGS kernel holds before handing an integral to
+        the 1000-interval kernel (1..16); results do not depend on it."""
+        _check(load().upc_ctx_set_qags_cap(self._ctx, int(cap)), "upc_ctx_set_qags_cap")
 
     def set_timing(self, enable=True):
         _check(load().upc_ctx_set_timing(self._ctx, int(enable)), "upc_ctx_set_timing")

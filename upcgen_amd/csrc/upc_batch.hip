@@ -151,7 +151,7 @@ struct upc_batch {
   int* d_tasks = nullptr;
   int* d_overflow = nullptr;
   unsigned long long* d_counters = nullptr;  // [0] evals [1] bad [2] overflow n [4] tasks [5] cursor
-  Qags<kQagsBigCap>* d_big = nullptr;
+  QagsBigArrays* d_big = nullptr;
   int nbig = 256;
   int qags_choice = 2, lumi_choice = 1;  // indices into the candidate lists
   hipStream_t cap_stream = nullptr;
@@ -204,7 +204,8 @@ int enqueue(upc_batch* b, hipStream_t s)
   BatchTask ts{b->d_tasks, b->d_row_k, b->d_row_logd, b->d_jobs, b->d_row0, nj};
   const int qblocks = b->ncu * kQagsBlockChoices[b->qags_choice];
   hipLaunchKernelGGL((k_flux_qags<kQagsCap, BatchTask>), dim3(qblocks), dim3(256), 0, s, ts, b->d_counters + 4,
-                     b->d_counters + 5, b->d_flux, (int*)nullptr, b->d_counters, b->d_overflow);
+                     b->d_counters + 5, b->d_flux, (int*)nullptr, b->d_counters, b->d_overflow,
+                     kQagsCap);
   hipLaunchKernelGGL((k_flux_qags_big<BatchTask>), dim3(b->nbig / 64), dim3(64), 0, s, ts, b->d_overflow,
                      b->d_counters, b->d_flux, (int*)nullptr, b->d_big, b->nbig);
   const int li = b->lumi_choice, w = kLumiWaveChoices[li];
@@ -403,7 +404,8 @@ int upc_batch_autotune(upc_batch* b, void* stream, upc_tune* h_out)
     UPC_HIP(hipEventRecord(e0, s));
     const int qblocks = b->ncu * kQagsBlockChoices[i];
     hipLaunchKernelGGL((k_flux_qags<kQagsCap, BatchTask>), dim3(qblocks), dim3(256), 0, s, ts, b->d_counters + 4,
-                       b->d_counters + 5, b->d_flux, (int*)nullptr, b->d_counters, b->d_overflow);
+                       b->d_counters + 5, b->d_flux, (int*)nullptr, b->d_counters, b->d_overflow,
+                     kQagsCap);
     UPC_HIP(hipEventRecord(e1, s));
     UPC_HIP(hipEventSynchronize(e1));
     float ms = 0;

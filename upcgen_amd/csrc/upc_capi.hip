@@ -159,6 +159,14 @@ int upc_ctx_set_mfma_spacing(upc_ctx* c, double h_fm)
   return mfma_set_spacing(c, h_fm);
 }
 
+int upc_ctx_set_qags_cap(upc_ctx* c, int cap)
+{
+  if (!c) return set_error("null context"), UPC_EINVAL;
+  if (cap < 1 || cap > kQagsCap) return set_error("QAGS fast-path cap must be in [1, 16]"), UPC_EINVAL;
+  c->qags_cap = cap;
+  return 0;
+}
+
 int upc_ctx_set_timing(upc_ctx* c, int enable)
 {
   if (!c) return set_error("null context"), UPC_EINVAL;

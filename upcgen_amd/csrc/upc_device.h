@@ -458,60 +458,6 @@ __device__ __forceinline__ QK21Out qk21_lds(const F& f, double a, double b, doub
   return o;
 }
 
-// dqpsrt on 1-based arrays.
-__device__ __forceinline__ void qpsrt(int limit, int last, int* maxerr, double* ermax, const double* elist,
-                             int* iord, int* nrmax)
-{
-  int i, k, isucc, jupbn, jbnd, ibeg;
-  double errmax, errmin;
-  if (last <= 2) {
-    iord[1] = 1;
-    iord[2] = 2;
-    goto L90;
-  }
-  errmax = elist[*maxerr];
-  if (*nrmax != 1) {
-    const int ido = *nrmax - 1;
-    for (i = 1; i <= ido; ++i) {
-      isucc = iord[*nrmax - 1];
-      if (errmax <= elist[isucc]) break;
-      iord[*nrmax] = isucc;
-      (*nrmax)--;
-    }
-  }
-  jupbn = last;
-  if (last > (limit / 2 + 2)) jupbn = limit + 3 - last;
-  errmin = elist[last];
-  jbnd = jupbn - 1;
-  ibeg = *nrmax + 1;
-  if (ibeg <= jbnd) {
-    for (i = ibeg; i <= jbnd; ++i) {
-      isucc = iord[i];
-      if (errmax >= elist[isucc]) goto L60;
-      iord[i - 1] = isucc;
-    }
-  }
-  iord[jbnd] = *maxerr;
-  iord[jupbn] = last;
-  goto L90;
-L60:
-  iord[i - 1] = *maxerr;
-  k = jbnd;
-  for (int j = i; j <= jbnd; ++j) {
-    isucc = iord[k];
-    if (errmin < elist[isucc]) {
-      iord[k + 1] = last;
-      goto L90;
-    }
-    iord[k + 1] = isucc;
-    k--;
-  }
-  iord[i] = last;
-L90:
-  *maxerr = iord[*nrmax];
-  *ermax = elist[*maxerr];
-}
-
 // dqelg (Wynn epsilon) on a 1-based table of length >= 55.
 __device__ __forceinline__ void qelg(int* n, double* epstab, double* result, double* abserr, double* res3la,
                             int* nres)

@@ -66,9 +66,6 @@ __device__ __forceinline__ void node_k(double b1, double b2, const double* cosk,
   *kp = p;
 }
 
-constexpr int kQagsCap = 16;       // fast per-lane interval list (the flux integrals stop by 15)
-constexpr int kQagsLimit = 1000;   // gsl_integration_qags limit of the reference
-constexpr int kQagsBigCap = 1000;
 
 // fluxPoint (src/UpcCrossSection.cpp:166-178): g1 for both beams.
 __device__ __forceinline__ double flux_point(double b, double k, double g1, double factor)
@@ -219,10 +216,13 @@ template <int CAP, class TS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(UPC_QAGS_MIN_WAVES, 8))) void k_flux_qags(
     TS ts, const unsigned long long* __restrict__ ntasks_p, unsigned long long* __restrict__ cursor,
     double* __restrict__ flux, int* __restrict__ ier_out, unsigned long long* __restrict__ stats,
-    int* __restrict__ overflow)
+    int* __restrict__ overflow, int fast_cap)
 {
   __shared__ double fvbuf[20 * 256];
-  Qags<CAP> q;
+  typename QagsSmallStore<CAP>::Arrays lists;
+  Qags<QagsSmallStore<CAP>> q;
+  q.s.a = &lists;
+  q.s.lim = fast_cap < CAP ? fast_cap : CAP;
   const long ntasks = (long)*ntasks_p;
   const int lane = threadIdx.x & 63;
   double* fv = fvbuf + (threadIdx.x >> 6) * 20 * 64 + lane;
@@ -247,7 +247,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(UPC_QAGS_MI
           f.ff = ft.ff;
           f.b = ft.b;
           f.w2g2 = ft.k * ft.k / ft.g1 / ft.g1;
-          q.start(0., 10., 1e-4, 1e-4, kQagsLimit);
+          q.start();
         } else {
           task = LONG_MAX;
         }
@@ -289,14 +289,15 @@ template <class TS>
 __global__ __launch_bounds__(64) void k_flux_qags_big(TS ts, const int* __restrict__ overflow,
                                                       unsigned long long* __restrict__ stats,
                                                       double* __restrict__ flux, int* __restrict__ ier_out,
-                                                      Qags<kQagsBigCap>* __restrict__ ws, int nws)
+                                                      QagsBigArrays* __restrict__ ws, int nws)
 {
   const long n = (long)stats[2];
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= nws) return;
   __shared__ double fvbuf[20 * 64];
   double* fv = fvbuf + threadIdx.x;
-  Qags<kQagsBigCap>& q = ws[gid];
+  Qags<QagsBigStore> q;
+  q.s.a = ws + gid;
   FluxFormIntegrand f;
   for (long i = gid; i < n; i += nws) {
     FluxTask ft;
@@ -304,7 +305,7 @@ __global__ __launch_bounds__(64) void k_flux_qags_big(TS ts, const int* __restri
     f.ff = ft.ff;
     f.b = ft.b;
     f.w2g2 = ft.k * ft.k / ft.g1 / ft.g1;
-    q.start(0., 10., 1e-4, 1e-4, kQagsLimit);
+    q.start();
     int st = kQagsRunning;
     while (st == kQagsRunning) {
       double qa, qb;

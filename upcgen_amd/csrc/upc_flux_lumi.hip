@@ -104,14 +104,14 @@ void free_work(upc_ctx* c)
   w = LumiWork{};
 }
 
-static Qags<kQagsBigCap>* g_big_ws = nullptr;  // lazily allocated, process lifetime
+static QagsBigArrays* g_big_ws = nullptr;  // lazily allocated, process lifetime
 static int g_big_ws_n = 0;
 
-static int big_ws(int n, Qags<kQagsBigCap>** out)
+static int big_ws(int n, QagsBigArrays** out)
 {
   if (g_big_ws_n < n) {
     if (g_big_ws) (void)hipFree(g_big_ws);
-    UPC_HIP(hipMalloc(&g_big_ws, sizeof(Qags<kQagsBigCap>) * n));
+    UPC_HIP(hipMalloc(&g_big_ws, sizeof(QagsBigArrays) * n));
     g_big_ws_n = n;
   }
   *out = g_big_ws;
@@ -150,8 +150,8 @@ static int run_flux(upc_ctx* c, int npts, int mirror_ny, hipStream_t s)
     GridTask ts{w.d_tasks, w.d_row_k, w.d_row_logd, fp, c->d_ff};
     hipLaunchKernelGGL((k_flux_qags<kQagsCap, GridTask>), dim3(qags_persistent_blocks()), dim3(256), 0, s, ts,
                        w.d_counters + 4, w.d_counters + 5, w.d_flux, (int*)nullptr, w.d_counters,
-                       w.d_overflow);
-    Qags<kQagsBigCap>* ws = nullptr;
+                       w.d_overflow, c->qags_cap);
+    QagsBigArrays* ws = nullptr;
     int rc = big_ws(256, &ws);
     if (rc) return rc;
     hipLaunchKernelGGL((k_flux_qags_big<GridTask>), dim3(4), dim3(64), 0, s, ts, w.d_overflow, w.d_counters,
@@ -368,8 +368,9 @@ int launch_flux_form(upc_ctx* c, const double* d_b, const double* d_k, int n, do
                      w.d_tasks, w.d_counters);
   PointTask ts{d_b, d_k, w.d_tasks, fp, c->d_ff};
   hipLaunchKernelGGL((k_flux_qags<kQagsCap, PointTask>), dim3(qags_persistent_blocks()), dim3(256), 0, s, ts,
-                     w.d_counters + 4, w.d_counters + 5, d_out, d_ier, w.d_counters, w.d_overflow);
-  Qags<kQagsBigCap>* ws = nullptr;
+                     w.d_counters + 4, w.d_counters + 5, d_out, d_ier, w.d_counters, w.d_overflow,
+                     c->qags_cap);
+  QagsBigArrays* ws = nullptr;
   rc = big_ws(256, &ws);
   if (rc) return rc;
   hipLaunchKernelGGL((k_flux_qags_big<PointTask>), dim3(4), dim3(64), 0, s, ts, w.d_overflow, w.d_counters,

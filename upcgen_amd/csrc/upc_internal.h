@@ -22,6 +22,7 @@ struct DevScalars {
   int pad;
 };
 
+constexpr int kQagsCap = 16;  // fast per-lane QAGS interval list (the flux integrals stop by 15)
 constexpr int kMaxEE = 1024;  // calcBreakupProb energy table capacity (reference: 10001)
 
 // Breakup knots actually needed: the luminosity reads P(min(b, 20)); the
@@ -99,6 +100,7 @@ struct upc_ctx {
   size_t spline_scratch_elems = 0;
   upc::LumiWork work;
   long long last_qags_calls = 0, last_qags_bad = 0, last_qags_evals = 0;
+  int qags_cap = upc::kQagsCap;  // intervals of the fast QAGS path before the 1000-interval kernel
   // MFMA / surrogate state is attached by their translation units.
   void* mfma = nullptr;
   void* surrogate = nullptr;

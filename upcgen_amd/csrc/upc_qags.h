@@ -1,9 +1,19 @@
 // Per-lane QUADPACK dqagse as a resumable state machine: every call to
 // consume() takes one QK21 result, so a wavefront can run one QK21 for all of
-// its lanes in lockstep while each lane advances its own adaptive integral
-// (the bookkeeping, which differs per lane, is cheap next to the 21
-// integrand evaluations).  The control flow and arithmetic follow dqagse
-// exactly (Piessens et al. 1983), which gsl_integration_qags restates.
+// its lanes in lockstep while each lane advances its own adaptive integral.
+// The control flow and arithmetic follow dqagse exactly (Piessens et al.
+// 1983), which gsl_integration_qags restates.
+//
+// Storage.  The scalar state lives in registers; only the lists that are
+// indexed per lane (intervals, results, errors, the order list, the epsilon
+// table) go through a store policy:
+//  - QagsSmallStore<CAP> (fast path): per-lane private arrays, intervals
+//    packed into 32 bits, the order list in 4-bit fields of one register;
+//  - QagsBigStore (the reference's 1000-interval workspace, global memory).
+// Every flux integral runs over q in [0, 10]; bisection keeps each interval a
+// dyadic piece [10 m 2^-L, 10 (m + 1) 2^-L] of it, and those end points (and
+// the midpoints 0.5 (a + b) that dqagse computes) are exact in FP64, so the
+// packed form (L, m) expands to the same doubles the FP64 lists hold.
 #pragma once
 
 #include "upc_device.h"
@@ -12,52 +22,178 @@ namespace upc {
 
 enum QagsStatus { kQagsRunning = 0, kQagsDone = 1, kQagsOverflow = 2 };
 
-// CAP: interval-list capacity (1-based arrays).  The epsilon table holds at
-// most one entry per bisection (numrl2 <= last + 1) plus the two qelg works
-// past its end, and never more than dqelg's limexp = 50 (+2, 1-based).
-template <int CAP>
-struct Qags {
-  static constexpr int NEPS = CAP + 6 < 55 ? CAP + 6 : 55;
-  double alist[CAP + 1], blist[CAP + 1], rlist[CAP + 1], elist[CAP + 1];
-  int iord[CAP + 1];
-  double rlist2[NEPS], res3la[4];
-  double a, b, epsabs, epsrel;
-  double result, abserr, area, errsum, errbnd, errmax, erlarg, ertest, small, correc, defabs,
-      erlast, a1, b1, a2, b2, area1, error1, defab1;
-  int limit, ier, last, ierro, iroff1, iroff2, iroff3, ksgn, ktmin, maxerr, nrmax, nres, numrl2,
-      extrap, noext, phase;
+constexpr double kQagsA = 0.0, kQagsB = 10.0;  // integration range of every flux integral
+constexpr double kQagsEps = 1e-4;               // epsabs = epsrel of the reference
+constexpr int kQagsLimit = 1000;                // gsl_integration_qags limit of the reference
 
-  __device__ void start(double a_, double b_, double epsabs_, double epsrel_, int limit_)
+template <int CAP>
+struct QagsSmallStore {
+  static_assert(CAP >= 2 && CAP <= 16, "the order list packs 16 4-bit entries");
+  static constexpr int cap = CAP;
+  // The epsilon table holds at most one entry per bisection (numrl2 <= last
+  // + 1) plus the two qelg works past its end (1-based).
+  static constexpr int kEps = CAP + 6;
+  using Iv = unsigned;  // L << 26 | m
+  struct Arrays {
+    Iv iv[CAP];
+    double rl[CAP], el[CAP];
+    double eps[kEps], r3[4];
+  };
+  Arrays* a;
+  unsigned long long io;  // iord[k] - 1 in bits 4 (k - 1) .. 4 (k - 1) + 3
+  int lim = CAP;          // intervals held before the integral is handed on (<= CAP)
+
+  __device__ void reset() { io = 0ull; }
+  __device__ int limit() const { return lim; }
+
+  __device__ static Iv root() { return 0u; }
+  __device__ static double scale(Iv v) { return ldexp(kQagsB, -(int)(v >> 26)); }
+  __device__ static double lo(Iv v) { return (double)(v & 0x3ffffffu) * scale(v); }
+  __device__ static double hi(Iv v) { return (double)((v & 0x3ffffffu) + 1u) * scale(v); }
+  __device__ static double mid(Iv v) { return (double)(2u * (v & 0x3ffffffu) + 1u) * (0.5 * scale(v)); }
+  __device__ static double width(Iv v) { return scale(v); }
+  __device__ static Iv left(Iv v) { return (((v >> 26) + 1u) << 26) | (2u * (v & 0x3ffffffu)); }
+  __device__ static Iv right(Iv v) { return left(v) | 1u; }
+
+  __device__ Iv iv(int k) const { return a->iv[k - 1]; }
+  __device__ void set_iv(int k, Iv v) { a->iv[k - 1] = v; }
+  __device__ double rl(int k) const { return a->rl[k - 1]; }
+  __device__ void set_rl(int k, double v) { a->rl[k - 1] = v; }
+  __device__ double el(int k) const { return a->el[k - 1]; }
+  __device__ void set_el(int k, double v) { a->el[k - 1] = v; }
+  __device__ int io_at(int k) const { return (int)((io >> (4 * (k - 1))) & 15ull) + 1; }
+  __device__ void set_io(int k, int v)
   {
-    a = a_;
-    b = b_;
-    epsabs = epsabs_;
-    epsrel = epsrel_;
-    limit = limit_;
+    const int sh = 4 * (k - 1);
+    io = (io & ~(15ull << sh)) | ((unsigned long long)(v - 1) << sh);
+  }
+  __device__ double* eps() { return a->eps; }
+  __device__ double* r3() { return a->r3; }
+};
+
+// The reference's workspace: limit 1000, FP64 end points, one per lane in
+// global memory.
+struct QagsBigArrays {
+  double alist[kQagsLimit], blist[kQagsLimit], rlist[kQagsLimit], elist[kQagsLimit];
+  int iord[kQagsLimit];
+  double eps[55], r3[4];  // dqelg's limexp = 50 (+2, 1-based)
+};
+
+struct QagsBigStore {
+  static constexpr int cap = kQagsLimit;
+  static constexpr int kEps = 55;
+  using Iv = double2;  // (a, b)
+  QagsBigArrays* a;
+
+  __device__ void reset() {}
+  __device__ static int limit() { return cap; }
+  __device__ static Iv root() { return make_double2(kQagsA, kQagsB); }
+  __device__ static double lo(Iv v) { return v.x; }
+  __device__ static double hi(Iv v) { return v.y; }
+  __device__ static double mid(Iv v) { return 0.5 * (v.x + v.y); }
+  __device__ static double width(Iv v) { return fabs(v.y - v.x); }
+  __device__ static Iv left(Iv v) { return make_double2(v.x, mid(v)); }
+  __device__ static Iv right(Iv v) { return make_double2(mid(v), v.y); }
+
+  __device__ Iv iv(int k) const { return make_double2(a->alist[k - 1], a->blist[k - 1]); }
+  __device__ void set_iv(int k, Iv v)
+  {
+    a->alist[k - 1] = v.x;
+    a->blist[k - 1] = v.y;
+  }
+  __device__ double rl(int k) const { return a->rlist[k - 1]; }
+  __device__ void set_rl(int k, double v) { a->rlist[k - 1] = v; }
+  __device__ double el(int k) const { return a->elist[k - 1]; }
+  __device__ void set_el(int k, double v) { a->elist[k - 1] = v; }
+  __device__ int io_at(int k) const { return a->iord[k - 1]; }
+  __device__ void set_io(int k, int v) { a->iord[k - 1] = v; }
+  __device__ double* eps() { return a->eps; }
+  __device__ double* r3() { return a->r3; }
+};
+
+// dqpsrt over a store (1-based indices).
+template <class S>
+__device__ __forceinline__ void qpsrt(S& s, int last, int* maxerr, double* ermax, int* nrmax)
+{
+  constexpr int limit = kQagsLimit;
+  int i, k, isucc, jupbn, jbnd, ibeg;
+  double errmax, errmin;
+  if (last <= 2) {
+    s.set_io(1, 1);
+    s.set_io(2, 2);
+    goto L90;
+  }
+  errmax = s.el(*maxerr);
+  if (*nrmax != 1) {
+    const int ido = *nrmax - 1;
+    for (i = 1; i <= ido; ++i) {
+      isucc = s.io_at(*nrmax - 1);
+      if (errmax <= s.el(isucc)) break;
+      s.set_io(*nrmax, isucc);
+      (*nrmax)--;
+    }
+  }
+  jupbn = last;
+  if (last > (limit / 2 + 2)) jupbn = limit + 3 - last;
+  errmin = s.el(last);
+  jbnd = jupbn - 1;
+  ibeg = *nrmax + 1;
+  if (ibeg <= jbnd) {
+    for (i = ibeg; i <= jbnd; ++i) {
+      isucc = s.io_at(i);
+      if (errmax >= s.el(isucc)) goto L60;
+      s.set_io(i - 1, isucc);
+    }
+  }
+  s.set_io(jbnd, *maxerr);
+  s.set_io(jupbn, last);
+  goto L90;
+L60:
+  s.set_io(i - 1, *maxerr);
+  k = jbnd;
+  for (int j = i; j <= jbnd; ++j) {
+    isucc = s.io_at(k);
+    if (errmin < s.el(isucc)) {
+      s.set_io(k + 1, last);
+      goto L90;
+    }
+    s.set_io(k + 1, isucc);
+    k--;
+  }
+  s.set_io(i, last);
+L90:
+  *maxerr = s.io_at(*nrmax);
+  *ermax = s.el(*maxerr);
+}
+
+template <class S>
+struct Qags {
+  using Iv = typename S::Iv;
+  static constexpr int limit = kQagsLimit;
+  S s;
+  Iv cur;  // the interval being bisected (the root before the first QK21)
+  double result, abserr, area, errsum, errbnd, errmax, erlarg, ertest, small, correc, defabs, erlast, area1,
+      error1, defab1;
+  int ier, last, ierro, iroff1, iroff2, iroff3, ksgn, ktmin, maxerr, nrmax, nres, numrl2, extrap, noext, phase;
+
+  __device__ void start()
+  {
+    s.reset();
+    cur = S::root();
     phase = 0;
   }
 
+  // Phase 0: the whole range; 1: left half of `cur`; 2: right half.
   __device__ __forceinline__ void interval(double* qa, double* qb) const
   {
-    if (phase == 0) {
-      *qa = a;
-      *qb = b;
-    } else if (phase == 1) {
-      *qa = a1;
-      *qb = b1;
-    } else {
-      *qa = a2;
-      *qb = b2;
-    }
+    *qa = phase == 2 ? S::mid(cur) : S::lo(cur);
+    *qb = phase == 1 ? S::mid(cur) : S::hi(cur);
   }
 
   __device__ int begin_bisection()
   {
-    if (last > CAP) return kQagsOverflow;
-    a1 = alist[maxerr];
-    b1 = 0.5 * (alist[maxerr] + blist[maxerr]);
-    a2 = b1;
-    b2 = blist[maxerr];
+    if (last > s.limit()) return kQagsOverflow;
+    cur = s.iv(maxerr);
     erlast = errmax;
     phase = 1;
     return kQagsRunning;
@@ -87,7 +223,7 @@ struct Qags {
   __device__ int finish_115()
   {
     result = 0.0;
-    for (int k = 1; k <= last; ++k) result = result + rlist[k];
+    for (int k = 1; k <= last; ++k) result = result + s.rl(k);
     abserr = errsum;
     return finish_130();
   }
@@ -106,18 +242,17 @@ struct Qags {
       defabs = o.resabs;
       const double resasc0 = o.resasc;
       const double dres = fabs(result);
-      errbnd = fmax(epsabs, epsrel * dres);
+      errbnd = fmax(kQagsEps, kQagsEps * dres);
       last = 1;
-      alist[1] = a;
-      blist[1] = b;
-      rlist[1] = result;
-      elist[1] = abserr;
-      iord[1] = 1;
+      s.set_iv(1, cur);
+      s.set_rl(1, result);
+      s.set_el(1, abserr);
+      s.set_io(1, 1);
       ier = 0;
       if (abserr <= 100.0 * epmach * defabs && abserr > errbnd) ier = 2;
       if (limit == 1) ier = 1;
       if (ier != 0 || (abserr <= errbnd && abserr != resasc0) || abserr == 0.0) return kQagsDone;
-      rlist2[1] = result;
+      s.eps()[1] = result;
       errmax = abserr;
       maxerr = 1;
       area = result;
@@ -145,48 +280,43 @@ struct Qags {
       return kQagsRunning;
     }
     // phase 2: body of the dqagse main loop for this value of `last`
+    const double a1 = S::lo(cur), b1 = S::mid(cur), a2 = b1, b2 = S::hi(cur);
     const double area2 = o.result, error2 = o.abserr, defab2 = o.resasc;
     const double area12 = area1 + area2;
     const double erro12 = error1 + error2;
+    const double rmax = s.rl(maxerr);
     errsum = errsum + erro12 - errmax;
-    area = area + area12 - rlist[maxerr];
+    area = area + area12 - rmax;
     if (!(defab1 == error1 || defab2 == error2)) {
-      if (!(fabs(rlist[maxerr] - area12) > 1e-5 * fabs(area12) || erro12 < 0.99 * errmax)) {
+      if (!(fabs(rmax - area12) > 1e-5 * fabs(area12) || erro12 < 0.99 * errmax)) {
         if (extrap) iroff2++;
         else iroff1++;
       }
       if (last > 10 && erro12 > errmax) iroff3++;
     }
-    rlist[maxerr] = area1;
-    rlist[last] = area2;
-    errbnd = fmax(epsabs, epsrel * fabs(area));
+    errbnd = fmax(kQagsEps, kQagsEps * fabs(area));
     if (iroff1 + iroff2 >= 10 || iroff3 >= 20) ier = 2;
     if (iroff2 >= 5) ierro = 3;
     if (last == limit) ier = 1;
     if (fmax(fabs(a1), fabs(b2)) <= (1.0 + 100.0 * epmach) * (fabs(a2) + 1000.0 * uflow)) ier = 4;
-    if (error2 > error1) {
-      alist[maxerr] = a2;
-      alist[last] = a1;
-      blist[last] = b1;
-      rlist[maxerr] = area2;
-      rlist[last] = area1;
-      elist[maxerr] = error2;
-      elist[last] = error1;
-    } else {
-      alist[last] = a2;
-      blist[maxerr] = b1;
-      blist[last] = b2;
-      elist[maxerr] = error1;
-      elist[last] = error2;
-    }
-    qpsrt(limit, last, &maxerr, &errmax, elist, iord, &nrmax);
+    // dqagse stores (a1, b1, area1, error1) at maxerr and (a2, b2, area2,
+    // error2) at last, then swaps them when error2 > error1: the final
+    // contents are written once here.
+    const bool swap = error2 > error1;
+    s.set_iv(maxerr, swap ? S::right(cur) : S::left(cur));
+    s.set_iv(last, swap ? S::left(cur) : S::right(cur));
+    s.set_rl(maxerr, swap ? area2 : area1);
+    s.set_rl(last, swap ? area1 : area2);
+    s.set_el(maxerr, swap ? error2 : error1);
+    s.set_el(last, swap ? error1 : error2);
+    qpsrt(s, last, &maxerr, &errmax, &nrmax);
     if (errsum <= errbnd) return finish_115();
     if (ier != 0) return finish_100();
     if (last == 2) {
-      small = fabs(b - a) * 0.375;
+      small = fabs(kQagsB - kQagsA) * 0.375;
       erlarg = errsum;
       ertest = errbnd;
-      rlist2[2] = area;
+      s.eps()[2] = area;
       ++last;
       return begin_bisection();
     }
@@ -195,9 +325,9 @@ struct Qags {
       return begin_bisection();
     }
     erlarg = erlarg - erlast;
-    if (fabs(b1 - a1) > small) erlarg = erlarg + erro12;
+    if (S::width(S::left(cur)) > small) erlarg = erlarg + erro12;
     if (!extrap) {
-      if (fabs(blist[maxerr] - alist[maxerr]) > small) {
+      if (S::width(s.iv(maxerr)) > small) {
         ++last;
         return begin_bisection();
       }
@@ -209,20 +339,20 @@ struct Qags {
       int jupbnd = last;
       if (last > (2 + limit / 2)) jupbnd = limit + 3 - last;
       for (int k = id; k <= jupbnd; ++k) {
-        maxerr = iord[nrmax];
-        errmax = elist[maxerr];
-        if (fabs(blist[maxerr] - alist[maxerr]) > small) {
+        maxerr = s.io_at(nrmax);
+        errmax = s.el(maxerr);
+        if (S::width(s.iv(maxerr)) > small) {
           ++last;
           return begin_bisection();
         }
         nrmax++;
       }
     }
-    if (numrl2 + 3 > NEPS - 1) return kQagsOverflow;  // small tables only
+    if (numrl2 + 3 > S::kEps - 1) return kQagsOverflow;  // small tables only
     numrl2++;
-    rlist2[numrl2] = area;
+    s.eps()[numrl2] = area;
     double reseps, abseps;
-    qelg(&numrl2, rlist2, &reseps, &abseps, res3la, &nres);
+    qelg(&numrl2, s.eps(), &reseps, &abseps, s.r3(), &nres);
     ktmin++;
     if (ktmin > 5 && abserr < 1e-3 * errsum) ier = 5;
     if (abseps < abserr) {
@@ -230,13 +360,13 @@ struct Qags {
       abserr = abseps;
       result = reseps;
       correc = erlarg;
-      ertest = fmax(epsabs, epsrel * fabs(reseps));
+      ertest = fmax(kQagsEps, kQagsEps * fabs(reseps));
       if (abserr <= ertest) return finish_100();
     }
     if (numrl2 == 1) noext = 1;
     if (ier == 5) return finish_100();
-    maxerr = iord[1];
-    errmax = elist[maxerr];
+    maxerr = s.io_at(1);
+    errmax = s.el(maxerr);
     nrmax = 1;
     extrap = 0;
     small = small * 0.5;
