@@ -125,6 +125,10 @@ int upc_flux_point(upc_ctx *ctx, const double *d_b, const double *d_k, int n, do
  * d_ier (may be NULL) receives the QAGS status per point. */
 int upc_flux_form(upc_ctx *ctx, const double *d_b, const double *d_k, int n, double *d_out,
                   int *d_ier, void *stream);
+/* J1(x), x >= 0, exactly as the form-factor flux integrand evaluates it on
+ * the device (role of gsl_sf_bessel_J1 in fluxForm, src/UpcCrossSection.cpp:189).
+ * No context needed. */
+int upc_bessel_j1(const double *d_x, int n, double *d_out, void *stream);
 /* UpcCrossSection::calcFormFac(Q2) (tabulated = 0) or its spline (1)
  *                                              src/UpcCrossSection.cpp:436-461 */
 int upc_form_factor(upc_ctx *ctx, const double *d_q2, int n, int tabulated, double *d_out,

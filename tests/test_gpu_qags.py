@@ -6,6 +6,8 @@ per lane, with interval end points packed as dyadic (level, index) pairs of
 1000-interval FP64 workspace in global memory.  Lowering the fast kernel's
 cap hands integrals to the second kernel part-way, so every cap must give
 bit-identical results, and both must match the oracle's QAGS."""
+import os
+
 import numpy as np
 import pytest
 
@@ -57,3 +59,16 @@ def test_qags_cap_range(gpu):
         for bad in (0, 17):
             with pytest.raises(gpu.UpcError):
                 g.set_qags_cap(bad)
+
+
+def test_device_j1_vs_mpmath(gpu):
+    """The device J1 (Chebyshev series below x = 8, Hankel modulus/phase
+    above, coefficients from gen_coeffs.py) against 50-digit mpmath values
+    over the integrand's range x = b q / hbar c <= 700."""
+    kat = np.load(os.path.join(os.path.dirname(__file__), "golden", "bessel_kat.npz"))
+    x, want = kat["xj"], kat["j1"]
+    got = gpu.bessel_j1(x).cpu().numpy()
+    env = np.where(x < 2, 1.0, np.sqrt(2 / (np.pi * np.maximum(x, 1e-300))))
+    err = np.abs(got - want) / env
+    print(f"max |J1 - mpmath| / envelope = {err.max():.2e} at x = {x[err.argmax()]:.4g}")
+    assert err.max() <= 5e-16

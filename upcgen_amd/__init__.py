@@ -11,7 +11,8 @@ import ctypes as C
 import os
 
 __all__ = ["Config", "Info", "Context", "load", "FORM_FP64", "FORM_MFMA", "FORM_SURROGATE", "FORM_MFMA_FP32",
-           "UpcError", "LIB_PATH", "Batch", "Tune", "BatchStats", "mixed_systems_jobs", "wood_saxon_radius"]
+           "UpcError", "LIB_PATH", "Batch", "Tune", "BatchStats", "mixed_systems_jobs", "wood_saxon_radius",
+           "bessel_j1"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libupcgen_amd.so")
@@ -99,6 +100,7 @@ SIGNATURES = {
     "upc_ctx_set_qags_cap": (_i, [_vp, _i]),
     "upc_flux_point": (_i, [_vp, _vp, _vp, _i, _vp, _vp]),
     "upc_flux_form": (_i, [_vp, _vp, _vp, _i, _vp, _vp, _vp]),
+    "upc_bessel_j1": (_i, [_vp, _i, _vp, _vp]),
     "upc_form_factor": (_i, [_vp, _vp, _i, _i, _vp, _vp]),
     "upc_breakup_prob": (_i, [_vp, _vp, _i, _i, _i, _vp, _vp]),
     "upc_nuclear_overlap": (_i, [_vp, _vp, _i, _i, _vp, _vp]),
@@ -402,6 +404,15 @@ class Context:
                                         _dptr(sum_zp), _dptr(ratio), seed, offset, n, _dptr(y), _dptr(m),
                                         _dptr(z), _stream(stream)), "upc_sample_events")
         return y, m, z
+
+
+def bessel_j1(x, stream=None):
+    """J1 on the device, as the flux integrand evaluates it (float64 tensor)."""
+    import torch
+    x = torch.as_tensor(x, dtype=torch.float64, device="cuda").contiguous()
+    out = torch.empty_like(x)
+    _check(load().upc_bessel_j1(_dptr(x), x.numel(), _dptr(out), _stream(stream)), "upc_bessel_j1")
+    return out
 
 
 def wood_saxon_radius(A):

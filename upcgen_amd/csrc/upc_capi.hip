@@ -213,6 +213,12 @@ int upc_flux_form(upc_ctx* c, const double* d_b, const double* d_k, int n, doubl
   return launch_flux_form(c, d_b, d_k, n, d_out, d_ier, S(stream));
 }
 
+int upc_bessel_j1(const double* d_x, int n, double* d_out, void* stream)
+{
+  if (n < 0 || (n && (!d_x || !d_out))) return set_error("bad arguments"), UPC_EINVAL;
+  return launch_bessel_j1(d_x, n, d_out, S(stream));
+}
+
 int upc_form_factor(upc_ctx* c, const double* d_q2, int n, int tabulated, double* d_out, void* stream)
 {
   if (!c || n < 0 || (n && (!d_q2 || !d_out))) return set_error("bad arguments"), UPC_EINVAL;

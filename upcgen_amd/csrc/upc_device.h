@@ -63,11 +63,11 @@ __device__ __forceinline__ double clenshaw(const double (&c)[N], double u)
   const double u2 = 2.0 * u;
 #pragma unroll 4
   for (int k = N - 1; k >= 1; --k) {
-    const double t = c[k] + u2 * b1 - b2;
+    const double t = fma(u2, b1, c[k] - b2);
     b2 = b1;
     b1 = t;
   }
-  return c[0] + u * b1 - b2;
+  return fma(u, b1, c[0] - b2);
 }
 
 // Two Chebyshev series of the same length in the same variable.
@@ -79,15 +79,15 @@ __device__ __forceinline__ void clenshaw2(const double (&c)[N], const double (&d
   const double u2 = 2.0 * u;
 #pragma unroll 4
   for (int k = N - 1; k >= 1; --k) {
-    const double t = c[k] + u2 * b1 - b2;
+    const double t = fma(u2, b1, c[k] - b2);
     b2 = b1;
     b1 = t;
-    const double s = d[k] + u2 * e1 - e2;
+    const double s = fma(u2, e1, d[k] - e2);
     e2 = e1;
     e1 = s;
   }
-  *fc = c[0] + u * b1 - b2;
-  *fd = d[0] + u * e1 - e2;
+  *fc = fma(u, b1, c[0] - b2);
+  *fd = fma(u, e1, d[0] - e2);
 }
 
 template <int N>
@@ -100,8 +100,8 @@ __device__ __forceinline__ double horner(const double (&c)[N], double y)
 }
 
 // sin and cos of |x| < 1e5: Cody-Waite reduction by pi/2 split in three
-// doubles (each step one FMA), Taylor polynomials on |r| <= pi/4 (the first
-// omitted terms are below 1e-19 relative).
+// doubles (each step one FMA), then near-minimax polynomials in r^2 on
+// |r| <= pi/4 (kSinPoly, kCosPoly: absolute errors 1.4e-17 and 9e-19).
 __device__ __forceinline__ void sincos_medium(double x, double* s, double* c)
 {
   constexpr double kTwoOverPi = 0.6366197723675814;
@@ -111,32 +111,40 @@ __device__ __forceinline__ void sincos_medium(double x, double* s, double* c)
   r = fma(-n, P2, r);
   r = fma(-n, P3, r);
   const double r2 = r * r;
-  // sin r = r + r^3 (-1/3! + r^2 (1/5! - ...)), cos r = 1 - r^2/2 + r^4 (1/4! - ...)
-  double ps = -1.0 / 121645100408832000.0;          // -1/19!
-  ps = 1.0 / 355687428096000.0 + r2 * ps;            //  1/17!
-  ps = -1.0 / 1307674368000.0 + r2 * ps;             // -1/15!
-  ps = 1.0 / 6227020800.0 + r2 * ps;                 //  1/13!
-  ps = -1.0 / 39916800.0 + r2 * ps;                  // -1/11!
-  ps = 1.0 / 362880.0 + r2 * ps;                     //  1/9!
-  ps = -1.0 / 5040.0 + r2 * ps;                      // -1/7!
-  ps = 1.0 / 120.0 + r2 * ps;                        //  1/5!
-  ps = -1.0 / 6.0 + r2 * ps;                         // -1/3!
+  double ps = kSinPoly[kSinPoly_N - 1];
+#pragma unroll
+  for (int k = kSinPoly_N - 2; k >= 0; --k) ps = fma(r2, ps, kSinPoly[k]);
   const double sr = fma(r * r2, ps, r);
-  double pc = 1.0 / 2432902008176640000.0;           //  1/20!
-  pc = -1.0 / 6402373705728000.0 + r2 * pc;          // -1/18!
-  pc = 1.0 / 20922789888000.0 + r2 * pc;             //  1/16!
-  pc = -1.0 / 87178291200.0 + r2 * pc;               // -1/14!
-  pc = 1.0 / 479001600.0 + r2 * pc;                  //  1/12!
-  pc = -1.0 / 3628800.0 + r2 * pc;                   // -1/10!
-  pc = 1.0 / 40320.0 + r2 * pc;                      //  1/8!
-  pc = -1.0 / 720.0 + r2 * pc;                       // -1/6!
-  pc = 1.0 / 24.0 + r2 * pc;                         //  1/4!
+  double pc = kCosPoly[kCosPoly_N - 1];
+#pragma unroll
+  for (int k = kCosPoly_N - 2; k >= 0; --k) pc = fma(r2, pc, kCosPoly[k]);
   const double cr = fma(r2 * r2, pc, fma(-0.5, r2, 1.0));
   const int q = (int)((long long)n & 3);
   const double ss = (q & 1) ? cr : sr;
   const double cc = (q & 1) ? sr : cr;
   *s = (q & 2) ? -ss : ss;
   *c = ((q + 1) & 2) ? -cc : cc;
+}
+
+// 1/x for normal x: the hardware estimate refined by two Newton steps
+// r += r (1 - x r) (each squares the relative error).
+__device__ __forceinline__ double rcp_nr(double x)
+{
+  double r = __builtin_amdgcn_rcp(x);
+  r = fma(r, fma(-x, r, 1.0), r);
+  r = fma(r, fma(-x, r, 1.0), r);
+  return r;
+}
+
+// 1/sqrt(x) for normal x > 0: the hardware estimate refined by two Newton
+// steps y += y (1/2 - (x/2) y^2) (each squares the relative error).
+__device__ __forceinline__ double rsqrt_nr(double x)
+{
+  double y = __builtin_amdgcn_rsq(x);
+  const double hx = 0.5 * x;
+  y = fma(y, fma(-hx * y, y, 0.5), y);
+  y = fma(y, fma(-hx * y, y, 0.5), y);
+  return y;
 }
 
 // J1(x), x >= 0 (role of gsl_sf_bessel_J1 in src/UpcCrossSection.cpp:189).
@@ -151,14 +159,16 @@ __device__ __forceinline__ double bessel_j1_small(double x)
 
 __device__ __forceinline__ double bessel_j1_large(double x)
 {
-  const double r = 1.0 / x;
-  const double v = 128.0 * r * r - 1.0;
+  constexpr double kInvSqrtPi = 0.56418958354775628695;
+  const double y = rsqrt_nr(x);
+  const double r = y * y;  // 1/x
+  const double v = fma(128.0 * r, r, -1.0);
   double P, Q;
   clenshaw2(kJ1P, kJ1Q, v, &P, &Q);
   Q *= 8.0 * r;
   double s, c;
   sincos_medium(x, &s, &c);
-  return (P * (s - c) + Q * (s + c)) / sqrt(kPi * x);
+  return (P * (s - c) + Q * (s + c)) * (y * kInvSqrtPi);
 }
 
 __device__ __forceinline__ double bessel_j1(double x)
@@ -264,26 +274,28 @@ __device__ __forceinline__ double bessel_k1_as(double x)
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ double knot(double x0, double dx, int i)
 {
-  return __dadd_rn(x0, __dmul_rn((double)i, dx));
+#pragma clang fp contract(off)
+  return x0 + (double)i * dx;
 }
 
+// The estimate (v - x0) / dx and the knot positions i dx each carry a
+// rounding error below 1e-9 of a knot spacing for the tables here (dx >=
+// 1e-6 with |x| <= 1e3), so the estimated interval is off by at most one:
+// one conditional step down, then one up, gives x_i <= v < x_{i+1}.
 __device__ __forceinline__ int spline_interval(double x0, double dx, double inv_dx, int n, double v,
                                                double* xi)
 {
   int i = (int)((v - x0) * inv_dx);
   i = i < 0 ? 0 : (i > n - 2 ? n - 2 : i);
   double xl = knot(x0, dx, i);
-  while (i > 0 && v < xl) {
-    --i;
-    xl = knot(x0, dx, i);
-  }
-  while (i < n - 2) {
-    const double xh = knot(x0, dx, i + 1);
-    if (v < xh) break;
-    ++i;
-    xl = xh;
-  }
-  *xi = xl;
+  const bool down = i > 0 && v < xl;
+  const double xd = knot(x0, dx, i - 1);
+  i = down ? i - 1 : i;
+  xl = down ? xd : xl;
+  const double xh = knot(x0, dx, i + 1);
+  const bool up = i < n - 2 && !(v < xh);
+  i = up ? i + 1 : i;
+  *xi = up ? xh : xl;
   return i;
 }
 

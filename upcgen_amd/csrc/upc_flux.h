@@ -130,7 +130,7 @@ struct FluxFormIntegrand {
     const double j1 = bessel_j1(b * q * (1.0 / kHc));
     const double d = tc - xl;
     const double f = c.x + d * (c.y + d * (c.z + d * c.w));
-    return q * q * f / t * j1;
+    return q * q * f * rcp_nr(t) * j1;  // t >= (k / g1)^2 > 0
   }
 };
 

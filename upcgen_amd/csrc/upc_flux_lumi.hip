@@ -322,6 +322,21 @@ int launch_nuc_cs_y(upc_ctx* c, double mpart, const double* d_cs_pos, const doub
   return 0;
 }
 
+// ---- J1 as the flux integrand evaluates it (C-ABI upc_bessel_j1) ----
+__global__ void k_bessel_j1(const double* __restrict__ x, int n, double* __restrict__ out)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = bessel_j1(x[i]);
+}
+
+int launch_bessel_j1(const double* d_x, int n, double* d_out, hipStream_t s)
+{
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k_bessel_j1, dim3(nblk(n, 256)), dim3(256), 0, s, d_x, n, d_out);
+  UPC_HIP(hipGetLastError());
+  return 0;
+}
+
 // ---- batched per-point flux (C-ABI upc_flux_point / upc_flux_form) ----
 __global__ void k_flux_point_pts(const double* __restrict__ b, const double* __restrict__ k, int n,
                                  FluxParams fp, double* __restrict__ out)

@@ -146,6 +146,7 @@ int launch_flux_point(upc_ctx* c, const double* d_b, const double* d_k, int n, d
 int launch_flux_form(upc_ctx* c, const double* d_b, const double* d_k, int n, double* d_out,
                      int* d_ier, hipStream_t s);
 void free_work(upc_ctx* c);
+int launch_bessel_j1(const double* d_x, int n, double* d_out, hipStream_t s);
 int launch_photon_flux(upc_ctx* c, const double* d_M, const double* d_Y, int n, double* d_out, hipStream_t s);
 int launch_nuc_cs_y(upc_ctx* c, double mpart, const double* d_cs_pos, const double* d_cs_neg, double* d_cs,
                     double* d_ratio, double* h_tot, hipStream_t s);
