@@ -102,6 +102,16 @@ __device__ __forceinline__ double horner(const double (&c)[N], double y)
 // sin and cos of |x| < 1e5: Cody-Waite reduction by pi/2 split in three
 // doubles (each step one FMA), then near-minimax polynomials in r^2 on
 // |r| <= pi/4 (kSinPoly, kCosPoly: absolute errors 1.4e-17 and 9e-19).
+// a + b c as one three-address v_fma_f64.  For a Horner step with a
+// loop-invariant coefficient `a` the compiler otherwise emits v_fmac_f64
+// (accumulator tied to `a`) preceded by a copy of `a`.
+__device__ __forceinline__ double fma3(double b, double c, double a)
+{
+  double r;
+  asm("v_fma_f64 %0, %1, %2, %3" : "=v"(r) : "v"(b), "v"(c), "s"(a));
+  return r;
+}
+
 __device__ __forceinline__ void sincos_medium(double x, double* s, double* c)
 {
   constexpr double kTwoOverPi = 0.6366197723675814;
@@ -113,13 +123,13 @@ __device__ __forceinline__ void sincos_medium(double x, double* s, double* c)
   const double r2 = r * r;
   double ps = kSinPoly[kSinPoly_N - 1];
 #pragma unroll
-  for (int k = kSinPoly_N - 2; k >= 0; --k) ps = fma(r2, ps, kSinPoly[k]);
+  for (int k = kSinPoly_N - 2; k >= 0; --k) ps = fma3(r2, ps, kSinPoly[k]);
   const double sr = fma(r * r2, ps, r);
   double pc = kCosPoly[kCosPoly_N - 1];
 #pragma unroll
-  for (int k = kCosPoly_N - 2; k >= 0; --k) pc = fma(r2, pc, kCosPoly[k]);
+  for (int k = kCosPoly_N - 2; k >= 0; --k) pc = fma3(r2, pc, kCosPoly[k]);
   const double cr = fma(r2 * r2, pc, fma(-0.5, r2, 1.0));
-  const int q = (int)((long long)n & 3);
+  const int q = (int)n & 3;  // |n| < 2^31 for |x| < 1e5
   const double ss = (q & 1) ? cr : sr;
   const double cc = (q & 1) ? sr : cr;
   *s = (q & 2) ? -ss : ss;
