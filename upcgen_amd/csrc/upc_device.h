@@ -276,6 +276,15 @@ __device__ __forceinline__ double bessel_k1_as(double x)
           y * (-0.780353e-2 + y * (0.325614e-2 + y * (-0.68245e-3)))))));
 }
 
+// LDS written by some lanes of a wavefront, then read by others of the same
+// wavefront: order the accesses without a workgroup barrier.
+__device__ __forceinline__ void wave_sync_lds()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // ---------------------------------------------------------------------------
 // Natural cubic spline on knots x_i = x0 + i*dx (each op rounded separately,
 // as the host loop that built the knots did).  coef[i] = (y_i, b_i, c_i, d_i)
@@ -316,6 +325,21 @@ __device__ __forceinline__ double spline_eval(const double4* __restrict__ coef, 
   const int i = spline_interval(x0, dx, inv_dx, n, v, &xl);
   const double4 c = coef[i];
   const double d = v - xl;
+  return c.x + d * (c.y + d * (c.z + d * c.w));
+}
+
+// The same cubic on the estimated interval alone (no one-knot correction).
+// When v lies within rounding of a knot the estimate may name the
+// neighbouring interval; the spline is C2, so that cubic agrees at v to a
+// few ulp.  For sums of many spline values, where the exact interval choice
+// does not matter.
+__device__ __forceinline__ double spline_eval_near(const double4* __restrict__ coef, double x0, double dx,
+                                                   double inv_dx, int n, double v)
+{
+  int i = (int)((v - x0) * inv_dx);
+  i = i < 0 ? 0 : (i > n - 2 ? n - 2 : i);
+  const double4 c = coef[i];
+  const double d = v - knot(x0, dx, i);
   return c.x + d * (c.y + d * (c.z + d * c.w));
 }
 

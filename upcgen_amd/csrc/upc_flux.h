@@ -47,13 +47,14 @@ __device__ __forceinline__ void node_k(double b1, double b2, const double* cosk,
 #pragma unroll
   for (int k = 0; k < 5; ++k) {
     const double c = cosk[k];
-    const double b = sqrt(node_b2(b1, b2, POL ? -c : c));
-    double breakup = 1., gaa_v = 1.;
-    if (b < 20.) {
-      if (mode != 1) breakup = spline_eval(bk, kBkMin, kDBk, 1.0 / kDBk, kNBkUsed, b);
-      gaa_v = spline_eval(gaa, 0.0, db, 1.0 / db, kNB, b);
-    } else if (mode != 1) {
-      breakup = p20;
+    const double x = node_b2(b1, b2, POL ? -c : c);
+    double breakup = mode != 1 ? p20 : 1., gaa_v = 1.;
+    if (x < 400.) {  // sqrt(x) >= 20 whenever x >= 400
+      const double b = sqrt(x);
+      if (b < 20.) {
+        if (mode != 1) breakup = spline_eval_near(bk, kBkMin, kDBk, 1.0 / kDBk, kNBkUsed, b);
+        gaa_v = spline_eval_near(gaa, 0.0, db, 1.0 / db, kNB, b);
+      }
     }
     if (POL) {  // w5[5 + k] = sin^2(phi_k)
       s += breakup * gaa_v * w5[k] * c * c;
@@ -405,27 +406,65 @@ __device__ __forceinline__ void lumi_fp64_body(const LumiView& v, int p0, LumiSm
   __syncthreads();
   // the extreme-cos node has the smallest b for every pair
   const double cext = POL ? -sm.cosk[0] : sm.cosk[4];
-  if (live) {
-    for (int i = lane; i < kNB1; i += 64) {
+  if (!live) return;
+  // Band of row i: the j with b^2 < 400 at the extreme cosine.  b^2 is convex
+  // in b2 (smallest at b2 = -b1 cext), so the band is the run of j around
+  // that vertex: one search for the vertex, one for each end.
+  int cnt[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int i = lane + 64 * h;
+    cnt[h] = 0;
+    if (i < kNB1) {
       const double b1 = sm.sb1[wv][i];
-      int lo = -1, hi = -1;
-      for (int j = 0; j < kNB2; ++j) {
-        if (node_b2(b1, sm.sb2[wv][j], cext) < 400.0) {
-          if (lo < 0) lo = j;
-          hi = j;
-        }
+      const double* b2s = sm.sb2[wv];
+      auto inb = [&](int j) { return node_b2(b1, b2s[j], cext) < 400.0; };
+      const double vtx = -b1 * cext;
+      int lo = 0, hi = kNB2;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (b2s[mid] < vtx) lo = mid + 1;
+        else hi = mid;
       }
-      sm.sjlo[wv][i] = lo < 0 ? 0 : lo;
-      sm.spre[wv][i + 1] = lo < 0 ? 0 : hi - lo + 1;
+      int jc = -1;
+      if (lo < kNB2 && inb(lo)) jc = lo;
+      if (lo > 0 && inb(lo - 1)) jc = lo - 1;
+      int jlo = 0;
+      if (jc >= 0) {
+        int a = 0, b = jc;  // first j in band: outside before it, inside from it to jc
+        while (a < b) {
+          const int mid = (a + b) >> 1;
+          if (inb(mid)) b = mid;
+          else a = mid + 1;
+        }
+        jlo = a;
+        a = jc;
+        b = kNB2 - 1;  // last j in band
+        while (a < b) {
+          const int mid = (a + b + 1) >> 1;
+          if (inb(mid)) a = mid;
+          else b = mid - 1;
+        }
+        cnt[h] = a - jlo + 1;
+      }
+      sm.sjlo[wv][i] = jlo;
     }
   }
-  __syncthreads();
-  if (live && lane == 0) {
-    sm.spre[wv][0] = 0;
-    for (int i = 1; i <= kNB1; ++i) sm.spre[wv][i] += sm.spre[wv][i - 1];
+  // inclusive wave scan of the two halves of the row counts
+  int c0 = cnt[0], c1 = cnt[1];
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int u0 = __shfl_up(c0, o), u1 = __shfl_up(c1, o);
+    if (lane >= o) {
+      c0 += u0;
+      c1 += u1;
+    }
   }
-  __syncthreads();
-  if (!live) return;
+  const int half = __shfl(c0, 63);
+  sm.spre[wv][lane + 1] = c0;
+  if (lane + 64 < kNB1) sm.spre[wv][lane + 65] = half + c1;
+  if (lane == 0) sm.spre[wv][0] = 0;
+  wave_sync_lds();
   // S0: the phi sum when every node is at or beyond 20 fm
   double s0s = 0, s0p = 0;
   {
@@ -442,14 +481,9 @@ __device__ __forceinline__ void lumi_fp64_body(const LumiView& v, int p0, LumiSm
   }
   const int T = sm.spre[wv][kNB1];
   double accs = 0, accp = 0;
+  int i = 0;  // row of pair t: pre[i] <= t < pre[i + 1]; t only grows
   for (int t = lane; t < T; t += 64) {
-    int lo = 0, hi = kNB1;  // largest i with pre[i] <= t
-    while (hi - lo > 1) {
-      const int mid = (lo + hi) >> 1;
-      if (sm.spre[wv][mid] <= t) lo = mid;
-      else hi = mid;
-    }
-    const int i = lo;
+    while (sm.spre[wv][i + 1] <= t) ++i;
     const int j = sm.sjlo[wv][i] + (t - sm.spre[wv][i]);
     double ks, kp;
     node_k<POL>(sm.sb1[wv][i], sm.sb2[wv][j], sm.cosk, sm.w5, v.mode, p20, sm.gaa, v.bk, &ks, &kp);
