@@ -247,9 +247,10 @@ int upc_batch_create(const upc_config* cfgs, int njobs, void* stream, upc_batch*
     const bool mirror = g.ymin == -g.ymax;
     pt0[j] = (int)np;
     row0[j] = (int)nr;
+    const int nwork = lumi_work_items((int)n, mirror ? g.ny : 0, mirror);
     for (int i = 0; i < 3; ++i) {
       blk0[i].push_back((int)blocks[i]);
-      blocks[i] += nblk(n, kLumiWaveChoices[i]);
+      blocks[i] += nblk(nwork, kLumiWaveChoices[i]);
     }
     np += n;
     nr += mirror ? n + g.nm : 2 * n;
@@ -301,6 +302,7 @@ int upc_batch_create(const upc_config* cfgs, int njobs, void* stream, upc_batch*
     J.v.npts = g.nm * g.ny;
     J.v.mode = g.breakup_mode;
     J.v.pol = g.use_polarized;
+    J.v.mirror_out = mirror;
     J.ff = c->d_ff;
     J.nm = g.nm;
     J.ny = g.ny;

@@ -347,7 +347,37 @@ struct LumiView {
   double* L;
   double* Ls;
   double* Lp;
+  // 0: one work item per point.  1 (symmetric grid, fp.mirror_ny = ny > 0):
+  // bins (m, iy) and (m, ny - iy) have the same M, dm, dy and exchanged flux
+  // rows, and the node sum is symmetric under b1 <-> b2, so one evaluation
+  // fills both; work items are the bins iy <= ny / 2.
+  int mirror_out;
 };
+
+// Work items of a contraction launch.
+__host__ __device__ inline int lumi_work_items(int npts, int mirror_ny, int mirror_out)
+{
+  return mirror_out ? npts / mirror_ny * (mirror_ny / 2 + 1) : npts;
+}
+
+// Work item q -> point p and its mirror bin pm (-1: none).
+__device__ __forceinline__ void work_point(int q, int mirror_ny, int mirror_out, int* p, int* pm)
+{
+  if (!mirror_out) {
+    *p = q;
+    *pm = -1;
+    return;
+  }
+  const int ny = mirror_ny, nc = ny / 2 + 1;
+  const int im = q / nc, iy = q - im * nc;
+  *p = im * ny + iy;
+  *pm = iy > 0 && ny - iy != iy ? im * ny + (ny - iy) : -1;
+}
+
+__device__ __forceinline__ void lumi_work_point(const LumiView& v, int q, int* p, int* pm)
+{
+  work_point(q, v.fp.mirror_ny, v.mirror_out, p, pm);
+}
 
 // Job resolvers: the job and first point of a block.
 struct SingleLumi {
@@ -380,8 +410,10 @@ __device__ __forceinline__ void lumi_fp64_body(const LumiView& v, int p0, LumiSm
     sm.w5[5 + threadIdx.x] = sn * sn;
   }
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int p = p0 + wv;
-  const bool live = p < v.npts;
+  const int q = p0 + wv;
+  const bool live = q < lumi_work_items(v.npts, v.fp.mirror_ny, v.mirror_out);
+  int p = 0, pm = -1;
+  if (live) lumi_work_point(v, q, &p, &pm);
   const FluxParams& fp = v.fp;
   const double p20 = v.sc->p20;
   double M = 0, dm = 1, dy = 1;
@@ -507,10 +539,17 @@ __device__ __forceinline__ void lumi_fp64_body(const LumiView& v, int p0, LumiSm
     const double tot_s = s0s * (sw1t * sw2t) + accs;
     if (POL) {
       const double tot_p = s0p * (sw1t * sw2t) + accp;
-      v.Ls[p] = pre * tot_s * dm * dy;
-      v.Lp[p] = pre * tot_p * dm * dy;
+      const double ls = pre * tot_s * dm * dy, lp = pre * tot_p * dm * dy;
+      v.Ls[p] = ls;
+      v.Lp[p] = lp;
+      if (pm >= 0) {
+        v.Ls[pm] = ls;
+        v.Lp[pm] = lp;
+      }
     } else {
-      v.L[p] = pre * tot_s * dm * dy;
+      const double l = pre * tot_s * dm * dy;
+      v.L[p] = l;
+      if (pm >= 0) v.L[pm] = l;
     }
   }
 }

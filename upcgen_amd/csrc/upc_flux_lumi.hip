@@ -221,11 +221,15 @@ int lumi_points_fp64(upc_ctx* c, const double* d_pts, int npts, double* d_L, dou
   int rc = prepare_flux(c, d_pts, npts, &fp, s);
   if (rc) return rc;
   constexpr int kWaves = 4;
+  const int mirror_out = fp.mirror_ny > 0;
   SingleLumi jr{LumiView{c->work.d_points, c->work.d_row_logd, c->work.d_flux, fp, c->d_gaa, c->d_bk,
-                         c->d_scalars, npts, c->cfg.breakup_mode, c->cfg.use_polarized, d_L, d_Ls, d_Lp}};
+                         c->d_scalars, npts, c->cfg.breakup_mode, c->cfg.use_polarized, d_L, d_Ls, d_Lp,
+                         mirror_out}};
+  const int nwork = lumi_work_items(npts, fp.mirror_ny, mirror_out);
   {
     PhaseScope ph(c, s, UPC_PHASE_LUMI);
-    hipLaunchKernelGGL((k_lumi_fp64<kWaves, SingleLumi>), dim3(nblk(npts, kWaves)), dim3(64 * kWaves), 0, s, jr);
+    hipLaunchKernelGGL((k_lumi_fp64<kWaves, SingleLumi>), dim3(nblk(nwork, kWaves)), dim3(64 * kWaves), 0, s,
+                       jr);
   }
   UPC_HIP(hipGetLastError());
   return read_stats(c, s);

@@ -280,15 +280,17 @@ __global__ __launch_bounds__(256) void k_mfma_project(int nrows, const double* _
 // One wave per point; lanes take (i, k) pairs.
 // ---------------------------------------------------------------------------
 template <bool POL>
-__global__ __launch_bounds__(256) void k_mfma_edge(int npts, const double* __restrict__ row_logd,
+__global__ __launch_bounds__(256) void k_mfma_edge(int npts, int mirror_out, const double* __restrict__ row_logd,
                                                    const double* __restrict__ flux, FluxParams fp,
                                                    const double* __restrict__ edge, double* __restrict__ out)
 {
   __shared__ double sb1[4][kNB1], sb2[4][kNB2], sw1[4][kNB1], sw2[4][kNB2], spre[4][kNB2 + 1];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int p0 = blockIdx.x * 4 + wv;
-  const bool live = p0 < npts;
-  const int p = live ? p0 : npts - 1;
+  const int nwork = lumi_work_items(npts, fp.mirror_ny, mirror_out);
+  const int q0 = blockIdx.x * 4 + wv;
+  const bool live = q0 < nwork;
+  int p, pm;  // the edge term of a mirror bin equals its partner's; only p is stored
+  work_point(live ? q0 : nwork - 1, fp.mirror_ny, mirror_out, &p, &pm);
   const int r2 = k2_row(p, npts, fp.mirror_ny);
   const double logd1 = row_logd[p], logd2 = row_logd[r2];
   for (int j = lane; j < kNB1; j += 64) {
@@ -400,7 +402,8 @@ __global__ __launch_bounds__(256) void k_mfma_edge(int npts, const double* __res
 // ---------------------------------------------------------------------------
 template <int PREC, bool POL>
 __global__ __launch_bounds__(64 * kMfmaWaves) void k_mfma_lumi(
-    const double* __restrict__ pts, int npts, FluxParams fp, MfmaGrid g, const uint16_t* __restrict__ Dh,
+    const double* __restrict__ pts, int npts, int mirror_out, FluxParams fp, MfmaGrid g,
+    const uint16_t* __restrict__ Dh,
     const uint16_t* __restrict__ Dl, const float* __restrict__ D32, const uint16_t* __restrict__ Wh,
     const uint16_t* __restrict__ Wl, const float* __restrict__ W32, const double* __restrict__ wscale,
     const double* __restrict__ wsum, const int* __restrict__ ulast, int mode, const DevScalars* __restrict__ sc,
@@ -408,8 +411,14 @@ __global__ __launch_bounds__(64 * kMfmaWaves) void k_mfma_lumi(
     unsigned long long* __restrict__ ksteps)
 {
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int p0 = (blockIdx.x * kMfmaWaves + wv) * 16;
-  if (p0 >= npts) return;
+  const int nwork = lumi_work_items(npts, fp.mirror_ny, mirror_out);
+  const int p0 = (blockIdx.x * kMfmaWaves + wv) * 16;  // first work item of the wave
+  if (p0 >= nwork) return;
+  auto point = [&](int q) {
+    int p, pm;
+    work_point(min(q, nwork - 1), fp.mirror_ny, mirror_out, &p, &pm);
+    return p;
+  };
   const int m = lane & 15, g4 = lane >> 4;
   double s0s = 0, s0p = 0;  // the phi sum beyond 20 fm, as in the FP64 form
   {
@@ -425,7 +434,7 @@ __global__ __launch_bounds__(64 * kMfmaWaves) void k_mfma_lumi(
       }
     }
   }
-  const int pa = min(p0 + m, npts - 1);
+  const int pa = point(p0 + m);
   const long rowA = (long)k2_row(pa, npts, fp.mirror_ny) * g.nu_pad;
   // extents of the 16 rows: A (k2 rows) bounds k, the k1 rows bound u
   int kend = ulast[k2_row(pa, npts, fp.mirror_ny)] + 1;
@@ -442,7 +451,7 @@ __global__ __launch_bounds__(64 * kMfmaWaves) void k_mfma_lumi(
   double dot_s[4] = {0, 0, 0, 0}, dot_p[4] = {0, 0, 0, 0};
   long rowC[4];
 #pragma unroll
-  for (int i = 0; i < 4; ++i) rowC[i] = (long)min(p0 + 4 * g4 + i, npts - 1) * g.nu_pad;
+  for (int i = 0; i < 4; ++i) rowC[i] = (long)point(p0 + 4 * g4 + i) * g.nu_pad;
   unsigned long long nk = 0;
   const int ntiles = (uend + 15) >> 4;
   const int kbmax = (kend + 31) >> 5;
@@ -503,16 +512,26 @@ __global__ __launch_bounds__(64 * kMfmaWaves) void k_mfma_lumi(
   if (m != 0) return;
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
-    const int q = p0 + 4 * g4 + i;
-    if (q >= npts) continue;
+    const int w = p0 + 4 * g4 + i;
+    if (w >= nwork) continue;
+    int q, qm;
+    work_point(w, fp.mirror_ny, mirror_out, &q, &qm);
     const int r2 = k2_row(q, npts, fp.mirror_ny);
     const double pre = 2 * kPi * kPi * pts[4 * q] * pts[4 * q + 2] * pts[4 * q + 3];
     const double ww = wsum[q] * wsum[r2], scale = wscale[q] * wscale[r2];
     if (POL) {
-      Ls[q] = pre * (s0s * ww + scale * dot_s[i] + edge_sum[q]);
-      Lp[q] = pre * (s0p * ww + scale * dot_p[i] + edge_sum[npts + q]);
+      const double ls = pre * (s0s * ww + scale * dot_s[i] + edge_sum[q]);
+      const double lp = pre * (s0p * ww + scale * dot_p[i] + edge_sum[npts + q]);
+      Ls[q] = ls;
+      Lp[q] = lp;
+      if (qm >= 0) {
+        Ls[qm] = ls;
+        Lp[qm] = lp;
+      }
     } else {
-      L[q] = pre * (s0s * ww + scale * dot_s[i] + edge_sum[q]);
+      const double l = pre * (s0s * ww + scale * dot_s[i] + edge_sum[q]);
+      L[q] = l;
+      if (qm >= 0) L[qm] = l;
     }
   }
 }
@@ -621,6 +640,8 @@ int lumi_points_mfma(upc_ctx* c, int prec, const double* d_pts, int npts, double
   rc = ensure_buffers(st, (size_t)nrows);
   if (rc) return rc;
   const bool split = prec == UPC_FORM_MFMA;
+  const int mirror_out = fp.mirror_ny > 0;
+  const int nwork = lumi_work_items(npts, fp.mirror_ny, mirror_out);
   {
     PhaseScope ph(c, s, UPC_PHASE_PROJECT);
     const long dn = (long)g.nu_pad * g.nu_pad;
@@ -639,18 +660,19 @@ int lumi_points_mfma(upc_ctx* c, int prec, const double* d_pts, int npts, double
                        c->work.d_flux, fp, g, split ? st->d_Wh : nullptr, split ? st->d_Wl : nullptr,
                        split ? nullptr : st->d_W32, st->d_wscale, st->d_wsum, st->d_ulast);
     if (pol)
-      hipLaunchKernelGGL(k_mfma_edge<true>, dim3(nblk(npts, 4)), dim3(256), 0, s, npts, c->work.d_row_logd,
-                         c->work.d_flux, fp, st->d_edge, st->d_edge_sum);
+      hipLaunchKernelGGL(k_mfma_edge<true>, dim3(nblk(nwork, 4)), dim3(256), 0, s, npts, mirror_out,
+                         c->work.d_row_logd, c->work.d_flux, fp, st->d_edge, st->d_edge_sum);
     else
-      hipLaunchKernelGGL(k_mfma_edge<false>, dim3(nblk(npts, 4)), dim3(256), 0, s, npts, c->work.d_row_logd,
-                         c->work.d_flux, fp, st->d_edge, st->d_edge_sum);
+      hipLaunchKernelGGL(k_mfma_edge<false>, dim3(nblk(nwork, 4)), dim3(256), 0, s, npts, mirror_out,
+                         c->work.d_row_logd, c->work.d_flux, fp, st->d_edge, st->d_edge_sum);
     UPC_HIP(hipMemsetAsync(st->d_ksteps, 0, sizeof(unsigned long long), s));
   }
   {
     PhaseScope ph(c, s, UPC_PHASE_LUMI);
-    const dim3 grid(nblk(npts, 16 * kMfmaWaves)), block(64 * kMfmaWaves);
-#define UPC_MFMA_LAUNCH(P, POLV)                                                                         \
-  hipLaunchKernelGGL((k_mfma_lumi<P, POLV>), grid, block, 0, s, c->work.d_points, npts, fp, g, st->d_Dh, \
+    const dim3 grid(nblk(nwork, 16 * kMfmaWaves)), block(64 * kMfmaWaves);
+#define UPC_MFMA_LAUNCH(P, POLV)                                                                   \
+  hipLaunchKernelGGL((k_mfma_lumi<P, POLV>), grid, block, 0, s, c->work.d_points, npts, mirror_out, fp, g, \
+                     st->d_Dh,                                                                     \
                      st->d_Dl, st->d_D32, st->d_Wh, st->d_Wl, st->d_W32, st->d_wscale, st->d_wsum,      \
                      st->d_ulast, c->cfg.breakup_mode, c->d_scalars, st->d_edge_sum, d_L, d_Ls, d_Lp,       \
                      st->d_ksteps)
