@@ -256,10 +256,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(UPC_QAGS_MI
     }
     if (__ballot(task != LONG_MAX) == 0) break;
     if (task != LONG_MAX) {
-      double qa, qb;
-      q.interval(&qa, &qb);
-      const QK21Out o = qk21_lds(f, qa, qb, fv, 64);
-      const int st = q.consume(o);
+      QK21Out o, o2{};
+      const int np = q.passes();
+#pragma unroll 1
+      for (int k = 0; k < np; ++k) {  // one copy of the integrand for both passes
+        double qa, qb;
+        q.interval(k, &qa, &qb);
+        const QK21Out r = qk21_lds(f, qa, qb, fv, 64);
+        if (k == 0) o = r;
+        else o2 = r;
+      }
+      const int st = q.consume(o, o2);
       if (st == kQagsDone) {
         const double Q = q.result / ft.A;
         flux[ft.out] = ft.factor * Q * Q / ft.k;
@@ -309,9 +316,17 @@ __global__ __launch_bounds__(64) void k_flux_qags_big(TS ts, const int* __restri
     q.start();
     int st = kQagsRunning;
     while (st == kQagsRunning) {
-      double qa, qb;
-      q.interval(&qa, &qb);
-      st = q.consume(qk21_lds(f, qa, qb, fv, 64));
+      QK21Out o, o2{};
+      const int np = q.passes();
+#pragma unroll 1
+      for (int k = 0; k < np; ++k) {
+        double qa, qb;
+        q.interval(k, &qa, &qb);
+        const QK21Out r = qk21_lds(f, qa, qb, fv, 64);
+        if (k == 0) o = r;
+        else o2 = r;
+      }
+      st = q.consume(o, o2);
     }
     const double Q = q.result / ft.A;
     flux[ft.out] = ft.factor * Q * Q / ft.k;

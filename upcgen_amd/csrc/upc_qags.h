@@ -1,6 +1,8 @@
 // Per-lane QUADPACK dqagse as a resumable state machine: every call to
-// consume() takes one QK21 result, so a wavefront can run one QK21 for all of
-// its lanes in lockstep while each lane advances its own adaptive integral.
+// consume() takes the QK21 results of one step (the whole range first, then
+// both halves of each bisection), so a wavefront can run the QK21 passes for
+// all of its lanes in lockstep while each lane advances its own adaptive
+// integral.
 // The control flow and arithmetic follow dqagse exactly (Piessens et al.
 // 1983), which gsl_integration_qags restates.
 //
@@ -172,8 +174,7 @@ struct Qags {
   static constexpr int limit = kQagsLimit;
   S s;
   Iv cur;  // the interval being bisected (the root before the first QK21)
-  double result, abserr, area, errsum, errbnd, errmax, erlarg, ertest, small, correc, defabs, erlast, area1,
-      error1, defab1;
+  double result, abserr, area, errsum, errbnd, errmax, erlarg, ertest, small, correc, defabs, erlast;
   int ier, last, ierro, iroff1, iroff2, iroff3, ksgn, ktmin, maxerr, nrmax, nres, numrl2, extrap, noext, phase;
 
   __device__ void start()
@@ -183,11 +184,13 @@ struct Qags {
     phase = 0;
   }
 
-  // Phase 0: the whole range; 1: left half of `cur`; 2: right half.
-  __device__ __forceinline__ void interval(double* qa, double* qb) const
+  // QK21 passes of the next step: phase 0 one (the whole range), phase 2
+  // two (k = 0: left half of `cur`, k = 1: right half).
+  __device__ __forceinline__ int passes() const { return phase == 0 ? 1 : 2; }
+  __device__ __forceinline__ void interval(int k, double* qa, double* qb) const
   {
-    *qa = phase == 2 ? S::mid(cur) : S::lo(cur);
-    *qb = phase == 1 ? S::mid(cur) : S::hi(cur);
+    *qa = phase != 0 && k == 1 ? S::mid(cur) : S::lo(cur);
+    *qb = phase != 0 && k == 0 ? S::mid(cur) : S::hi(cur);
   }
 
   __device__ int begin_bisection()
@@ -195,7 +198,7 @@ struct Qags {
     if (last > s.limit()) return kQagsOverflow;
     cur = s.iv(maxerr);
     erlast = errmax;
-    phase = 1;
+    phase = 2;
     return kQagsRunning;
   }
 
@@ -233,7 +236,9 @@ struct Qags {
     return kQagsDone;
   }
 
-  __device__ __forceinline__ int consume(const QK21Out& o)
+  // o: the pass on the whole range (phase 0) or the left half; o2: the right
+  // half (phase 2 only).
+  __device__ __forceinline__ int consume(const QK21Out& o, const QK21Out& o2)
   {
     constexpr double epmach = DBL_EPSILON, uflow = DBL_MIN, oflow = DBL_MAX;
     if (phase == 0) {
@@ -272,16 +277,10 @@ struct Qags {
       last = 2;
       return begin_bisection();
     }
-    if (phase == 1) {
-      area1 = o.result;
-      error1 = o.abserr;
-      defab1 = o.resasc;
-      phase = 2;
-      return kQagsRunning;
-    }
     // phase 2: body of the dqagse main loop for this value of `last`
     const double a1 = S::lo(cur), b1 = S::mid(cur), a2 = b1, b2 = S::hi(cur);
-    const double area2 = o.result, error2 = o.abserr, defab2 = o.resasc;
+    const double area1 = o.result, error1 = o.abserr, defab1 = o.resasc;
+    const double area2 = o2.result, error2 = o2.abserr, defab2 = o2.resasc;
     const double area12 = area1 + area2;
     const double erro12 = error1 + error2;
     const double rmax = s.rl(maxerr);
