@@ -173,8 +173,12 @@ __device__ __forceinline__ double bessel_j1_large(double x)
   const double y = rsqrt_nr(x);
   const double r = y * y;  // 1/x
   const double v = fma(128.0 * r, r, -1.0);
-  double P, Q;
-  clenshaw2(kJ1P, kJ1Q, v, &P, &Q);
+  double P = kJ1P[kJ1P_N - 1], Q = kJ1Q[kJ1Q_N - 1];
+#pragma unroll
+  for (int k = kJ1P_N - 2; k >= 0; --k) {
+    P = fma3(v, P, kJ1P[k]);
+    Q = fma3(v, Q, kJ1Q[k]);
+  }
   Q *= 8.0 * r;
   double s, c;
   sincos_medium(x, &s, &c);
