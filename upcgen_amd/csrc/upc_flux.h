@@ -120,16 +120,18 @@ struct FluxFormIntegrand {
   const double4* __restrict__ ff;
   double b, w2g2;
   // The spline record is a dependent load from a 32 MB table: it is issued
-  // first and J1 (independent ALU work) runs while it is in flight.
+  // first and J1 (independent ALU work) runs while it is in flight.  The
+  // interval is the rounded estimate alone, as in spline_eval_near: within
+  // rounding of a knot the neighbouring cubic agrees to a few ulp.
   __device__ __forceinline__ double operator()(double q) const
   {
     const double t = q * q + w2g2;
     const double tc = t < kQ2max ? t : kQ2max - kDQ2;
-    double xl;
-    const int i = spline_interval(kQ2min, kDQ2, 1.0 / kDQ2, kNQ2, tc, &xl);
+    int i = (int)((tc - kQ2min) * (1.0 / kDQ2));
+    i = i < 0 ? 0 : (i > kNQ2 - 2 ? kNQ2 - 2 : i);
     const double4 c = ff[i];
     const double j1 = bessel_j1(b * q * (1.0 / kHc));
-    const double d = tc - xl;
+    const double d = tc - knot(kQ2min, kDQ2, i);
     const double f = c.x + d * (c.y + d * (c.z + d * c.w));
     return q * q * f * rcp_nr(t) * j1;  // t >= (k / g1)^2 > 0
   }
