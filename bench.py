@@ -34,16 +34,17 @@ FP32_MFMA_PEAK_TFLOPS = 157.3
 KSTEP_FLOP = 2 * 16 * 16 * 32
 # FP64 operations of one form-factor flux integrand evaluation q^2 F(t)/t J1(bq/hc)
 # (fluxFormIntegrand, src/UpcCrossSection.cpp:181-191) plus its QK21 accumulation,
-# counted on the J1 branch x > 8 that most evaluations take (FMA = 2):
+# counted on the J1 branch x > 8 that most evaluations take (FMA = 2), as the
+# device evaluates it:
 #   t and the Q^2 clamp 2, cubic-spline lookup 11, q^2 F / t 3, x = b q / hc 2,
-#   J1 asymptotic form: two 18-term Chebyshev series 72, 1/x and argument 6,
-#   sin/cos with argument reduction 40, combination + sqrt + divide 8,
+#   J1 modulus/phase form: 1/sqrt(x) and 1/x 9, two 13-term polynomials 48,
+#   v and the Q scale 3, sin/cos with argument reduction 39, combination 6,
 #   product 1, QK21 Gauss/Kronrod/abs sums 8
-FLOP_PER_QAGS_EVAL = 153
+FLOP_PER_QAGS_EVAL = 132
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (one
 # counter per pass) on this workload, scaled by a 1 GiB copy's readings; the
 # commands and raw counts are in profiles/pmc_traffic_1001x121_xnxn.txt.
-TRAFFIC_BYTES = {"k_flux_qags": 1.724e11, "k_lumi_fp64": 2.357e8, "k_mfma_lumi_split": 5.527e9}
+TRAFFIC_BYTES = {"k_flux_qags": 5.130e10, "k_lumi_fp64": 1.300e8, "k_mfma_lumi_split": 2.124e9}
 HBM_PEAK_GBS = 8000.0
 
 
