@@ -30,6 +30,7 @@ __device__ void spline_window(XF&& xk, YF&& yk, int w0, int w1, int r0, int r1, 
                               double* gam, double* zd, int stride)
 {
   double off_prev = 0, gam_prev = 0, z_prev = 0;
+#pragma unroll 4
   for (int r = w0; r < w1; ++r) {
     const double x0 = xk(r), x1 = xk(r + 1), x2 = xk(r + 2);
     const double y0 = yk(r), y1 = yk(r + 1), y2 = yk(r + 2);
@@ -56,6 +57,7 @@ __device__ void spline_window(XF&& xk, YF&& yk, int w0, int w1, int r0, int r1, 
   }
   double cnext = zd[(size_t)(w1 - 1 - w0) * stride];
   if (w1 - 1 >= r0 && w1 - 1 < r1) c[w1] = cnext;
+#pragma unroll 4
   for (int r = w1 - 2; r >= w0; --r) {
     const double cr = zd[(size_t)(r - w0) * stride] - gam[(size_t)(r - w0) * stride] * cnext;
     if (r >= r0 && r < r1) c[r + 1] = cr;
@@ -328,7 +330,12 @@ __device__ __forceinline__ double breakup_prob(double b, int mode, double g1, co
   return P;
 }
 
-// P(b) at the breakup knots (d_b == nullptr) or at caller points.
+// P(b) at the breakup knots (d_b == nullptr) or at caller points.  The
+// trapezoid sum over the ~600 photon energies of one b is split over a group
+// of kBkLanes consecutive lanes (contiguous energy ranges, partial sums added
+// by shuffles): the same terms, summed in another order.
+constexpr int kBkLanes = 8;
+
 __global__ __launch_bounds__(256) void k_breakup_values(double g1, int mode, const double* __restrict__ ee_g,
                                                         const double* __restrict__ se_g,
                                                         const DevScalars* sc, int n,
@@ -342,10 +349,54 @@ __global__ __launch_bounds__(256) void k_breakup_values(double g1, int mode, con
     se[i] = se_g[i];
   }
   __syncthreads();
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int sub = gid % kBkLanes;
+  const int i0 = gid / kBkLanes;
+  const bool live = i0 < n;
+  const int i = live ? i0 : n - 1;
   const double b = d_b ? d_b[i] : knot(kBkMin, kDBk, i);
-  out[i] = breakup_prob(b, mode, g1, ee, se, sc->zcon, sc->o0);
+  // src/UpcCrossSection.cpp:972-1018, as breakup_prob()
+  const double hbarcmev = 197.3269718;
+  const double gammatarg = 2. * g1 * g1 - 1.;
+  const double omaxx = g1 > 500. ? 1.E10 : 1.E7;
+  const double omax = fmin(omaxx, 4. * gammatarg * (hbarcmev) / b);
+  const double scale = 1.0 / ((hbarcmev)*gammatarg);
+  const double zcon = sc->zcon;
+  // terms k = 2 .. kend - 1, kend = the first k with ee[k] >= omax (ee ascends)
+  int kend = 2;
+  {
+    int lo = 2, hi = nee;  // ee[nee] is the 1e11 sentinel
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (ee[mid] < omax) lo = mid + 1;
+      else hi = mid;
+    }
+    kend = lo;
+  }
+  const int nterm = kend - 2;
+  const int per = (nterm + kBkLanes - 1) / kBkLanes;
+  const int ka = 2 + sub * per, kb = min(kend, ka + per);
+  double part = 0.;
+  if (ka < kb) {
+    double gk1m = bessel_k1_as(ee[ka - 1] * b * scale);
+    for (int k = ka; k < kb; ++k) {
+      const double gk1 = bessel_k1_as(ee[k] * b * scale);
+      part = part + zcon * (ee[k] - ee[k - 1]) * .5 * (se[k - 1] * ee[k - 1] * gk1m * gk1m + se[k] * ee[k] * gk1 * gk1);
+      gk1m = gk1;
+    }
+  }
+#pragma unroll
+  for (int o = 1; o < kBkLanes; o <<= 1) part += __shfl_xor(part, o);
+  if (!live || sub != 0) return;
+  const double pxn = part;
+  double P = 0.;
+  if (omax >= sc->o0) {
+    if (mode == 1) P = 1.;
+    if (mode == 2) P = (1 - exp(-1 * pxn)) * (1 - exp(-1 * pxn));
+    if (mode == 3) P = exp(-2 * pxn);
+    if (mode == 4) P = 2. * exp(-pxn) * (1. - exp(-pxn));
+  }
+  out[i] = P;
 }
 
 __global__ void k_p20(const double4* __restrict__ bk, int mode, DevScalars* sc)
@@ -407,8 +458,10 @@ __global__ void k_eval_overlap(const double4* __restrict__ tab, const double* __
 // ---------------------------------------------------------------------------
 static inline int nblk(long n, int bs) { return (int)((n + bs - 1) / bs); }
 
-constexpr int kSplineChunk = 512;
-constexpr int kSplineGuard = 64;
+// Rows solved per thread, and the guard rows of its window on each side
+// ((2 - sqrt 3)^48 = 3e-28): short serial chains, 15.6 k threads for 10^6 knots.
+constexpr int kSplineChunk = 64;
+constexpr int kSplineGuard = 48;
 
 int alloc_tables(upc_ctx* c)
 {
@@ -466,7 +519,7 @@ int build_tables(upc_ctx* c, hipStream_t s)
   if (rc) return rc;
   hipLaunchKernelGGL(k_breakup_init, dim3(1), dim3(64), 0, s, c->g1, c->d_ee, c->d_se, c->d_scalars);
   if (p.breakup_mode > 1) {
-    hipLaunchKernelGGL(k_breakup_values, dim3(nblk(kNBkUsed, 256)), dim3(256), 0, s, c->g1,
+    hipLaunchKernelGGL(k_breakup_values, dim3(nblk((long)kNBkUsed * kBkLanes, 256)), dim3(256), 0, s, c->g1,
                        p.breakup_mode, c->d_ee, c->d_se, c->d_scalars, kNBkUsed, (const double*)nullptr,
                        c->d_bk_y);
     UPC_HIP(hipGetLastError());
@@ -504,8 +557,8 @@ int launch_breakup(upc_ctx* c, const double* d_b, int n, int mode, int tabulated
       set_error("breakup mode must be 1..4");
       return UPC_EINVAL;
     }
-    hipLaunchKernelGGL(k_breakup_values, dim3(nblk(n, 256)), dim3(256), 0, s, c->g1, mode, c->d_ee,
-                       c->d_se, c->d_scalars, n, d_b, d_out);
+    hipLaunchKernelGGL(k_breakup_values, dim3(nblk((long)n * kBkLanes, 256)), dim3(256), 0, s, c->g1, mode,
+                       c->d_ee, c->d_se, c->d_scalars, n, d_b, d_out);
   }
   UPC_HIP(hipGetLastError());
   return 0;

@@ -251,6 +251,99 @@ __global__ __launch_bounds__(256) void k_pdf_init(const double* __restrict__ bin
   }
 }
 
+// The same table for one long row (the 2-D (Y, M) pdf) over many blocks:
+// segment sums -> mean -> scaled segment sums -> per-segment offsets and the
+// prefix inside each segment.  Every block re-derives the mean and its offset
+// from the segment sums in one fixed order, so all blocks agree.
+constexpr int kPdfSeg = 2048;  // bins per block
+
+__global__ __launch_bounds__(256) void k_pdf_seg_sum(const double* __restrict__ row, int n,
+                                                     double* __restrict__ seg, int* __restrict__ bad)
+{
+  __shared__ double part[256];
+  const int s0 = blockIdx.x * kPdfSeg, s1 = min(n, s0 + kPdfSeg);
+  double acc = 0;
+  int neg = 0;
+  for (int i = s0 + threadIdx.x; i < s1; i += 256) {
+    acc += row[i];
+    neg |= row[i] < 0;
+  }
+  if (neg) atomicOr(bad, 1);
+  part[threadIdx.x] = acc;
+  __syncthreads();
+  for (int d = 128; d > 0; d >>= 1) {
+    if (threadIdx.x < d) part[threadIdx.x] += part[threadIdx.x + d];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) seg[blockIdx.x] = part[0];
+}
+
+__device__ double pdf_mean(const double* __restrict__ seg, int nseg, int n)
+{
+  double tot = 0;
+  for (int b = 0; b < nseg; ++b) tot += seg[b];
+  return tot / n;
+}
+
+__global__ __launch_bounds__(256) void k_pdf_seg_scaled(const double* __restrict__ row, int n,
+                                                        const double* __restrict__ seg, int nseg,
+                                                        double* __restrict__ seg_scaled)
+{
+  __shared__ double part[256];
+  __shared__ double mean_s;
+  if (threadIdx.x == 0) mean_s = pdf_mean(seg, nseg, n);
+  __syncthreads();
+  const double mean = mean_s;
+  const int s0 = blockIdx.x * kPdfSeg, s1 = min(n, s0 + kPdfSeg);
+  const int per = kPdfSeg / 256;
+  const int a = s0 + threadIdx.x * per, e = min(s1, a + per);
+  double acc = 0;
+  for (int i = a; i < e; ++i) acc += (row[i] / mean) / n;
+  part[threadIdx.x] = acc;
+  __syncthreads();
+  for (int d = 128; d > 0; d >>= 1) {
+    if (threadIdx.x < d) part[threadIdx.x] += part[threadIdx.x + d];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) seg_scaled[blockIdx.x] = part[0];
+}
+
+__global__ __launch_bounds__(256) void k_pdf_seg_apply(const double* __restrict__ row, int n,
+                                                       const double* __restrict__ seg, int nseg,
+                                                       const double* __restrict__ seg_scaled,
+                                                       double* __restrict__ sum)
+{
+  __shared__ double part[256];
+  __shared__ double mean_s, base_s;
+  if (threadIdx.x == 0) {
+    mean_s = pdf_mean(seg, nseg, n);
+    double base = 0;
+    for (int b = 0; b < (int)blockIdx.x; ++b) base += seg_scaled[b];
+    base_s = base;
+    if (blockIdx.x == 0) sum[0] = 0;
+  }
+  __syncthreads();
+  const double mean = mean_s;
+  const int s0 = blockIdx.x * kPdfSeg, s1 = min(n, s0 + kPdfSeg);
+  const int per = kPdfSeg / 256;
+  const int a = s0 + threadIdx.x * per, e = min(s1, a + per);
+  double acc = 0;
+  for (int i = a; i < e; ++i) acc += (row[i] / mean) / n;
+  part[threadIdx.x] = acc;
+  __syncthreads();
+  for (int d = 1; d < 256; d <<= 1) {
+    const double v = threadIdx.x >= d ? part[threadIdx.x - d] : 0;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  double base = base_s + (threadIdx.x ? part[threadIdx.x - 1] : 0);
+  for (int i = a; i < e; ++i) {
+    base += (row[i] / mean) / n;
+    sum[i + 1] = base;
+  }
+}
+
 // gsl_histogram_pdf_sample on a prefix table: sum[i] <= r < sum[i+1].
 __device__ __forceinline__ long pdf_find(const double* __restrict__ sum, long n, double r)
 {
@@ -393,7 +486,18 @@ int launch_pdf_init(const double* d_bins, int nrows, int n, double* d_sum, hipSt
   int* d_bad = nullptr;
   UPC_HIP(hipMallocAsync((void**)&d_bad, sizeof(int), s));
   UPC_HIP(hipMemsetAsync(d_bad, 0, sizeof(int), s));
-  hipLaunchKernelGGL(k_pdf_init, dim3(nrows), dim3(256), 0, s, d_bins, n, d_sum, d_bad);
+  const int nseg = (n + kPdfSeg - 1) / kPdfSeg;
+  double* d_seg = nullptr;
+  if (nrows == 1 && nseg > 1) {
+    UPC_HIP(hipMallocAsync((void**)&d_seg, sizeof(double) * 2 * nseg, s));
+    hipLaunchKernelGGL(k_pdf_seg_sum, dim3(nseg), dim3(256), 0, s, d_bins, n, d_seg, d_bad);
+    hipLaunchKernelGGL(k_pdf_seg_scaled, dim3(nseg), dim3(256), 0, s, d_bins, n, d_seg, nseg, d_seg + nseg);
+    hipLaunchKernelGGL(k_pdf_seg_apply, dim3(nseg), dim3(256), 0, s, d_bins, n, d_seg, nseg, d_seg + nseg,
+                       d_sum);
+    UPC_HIP(hipFreeAsync(d_seg, s));
+  } else {
+    hipLaunchKernelGGL(k_pdf_init, dim3(nrows), dim3(256), 0, s, d_bins, n, d_sum, d_bad);
+  }
   UPC_HIP(hipGetLastError());
   int bad = 0;
   UPC_HIP(hipMemcpyAsync(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost, s));
