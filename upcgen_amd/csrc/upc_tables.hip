@@ -91,19 +91,24 @@ __global__ void k_ws_rho0(double R, double a, int A, DevScalars* sc)
 }
 
 // ---------------------------------------------------------------------------
-// prepareGAA, src/UpcCrossSection.cpp:364-414 (one 256-thread block)
+// prepareGAA, src/UpcCrossSection.cpp:364-414, in three launches: T_A and its
+// spline (one block); the G_AA integral of each b (one block per b: the 200
+// integrand values in parallel, then the quirky Simpson sum in the reference's
+// order by one thread); the G_AA spline (one block).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_gaa(double R, double a, double sqrts, DevScalars* sc,
-                                             double4* out_ta, double4* out_gaa)
+__device__ __forceinline__ double gaa_cs_nn(double sqrts)
 {
-  __shared__ double vTA[kNB], vG[kNB], cc[kNB], gam[kNB], zd[kNB];
-  __shared__ double4 coefTA[kNB];
-  __shared__ double cosk[5];
-  const double bmax = 20.;
-  const double db = bmax / (kNB - 1);
+  const double ssm = pow(sqrts, 2) / pow(2 * kMProt + 2.1206, 2);
+  return 0.1 * (34.41 + 0.2720 * pow(log(ssm), 2) + 13.07 * pow(ssm, -0.4473) - 7.394 * pow(ssm, -0.5486));
+}
+
+__global__ __launch_bounds__(256) void k_gaa_ta(double R, double a, double sqrts, DevScalars* sc,
+                                                double4* out_ta)
+{
+  __shared__ double vTA[kNB], cc[kNB], gam[kNB], zd[kNB];
+  const double db = 20. / (kNB - 1);
   const int t = threadIdx.x;
   const double rho0 = sc->rho0;
-  if (t < 5) cosk[t] = cos(kPi * kGLx5[t]);
   if (t < kNB) {
     const double b = t * db;
     vTA[t] = 2. * simpson_fn(kNB, db, [&](int iz) {
@@ -118,37 +123,51 @@ __global__ __launch_bounds__(256) void k_gaa(double R, double a, double sqrts, D
     cc[0] = 0.0;
     cc[kNB - 1] = 0.0;
     spline_window(xk, [&](int i) { return vTA[i]; }, 0, kNB - 2, 0, kNB - 2, cc, gam, zd, 1);
+    sc->csNN = gaa_cs_nn(sqrts);
   }
   __syncthreads();
-  if (t < kNB - 1) {
-    coefTA[t] = spline_coef(xk(t), xk(t + 1), vTA[t], vTA[t + 1], cc[t], cc[t + 1]);
-    out_ta[t] = coefTA[t];
-  }
+  if (t < kNB - 1) out_ta[t] = spline_coef(xk(t), xk(t + 1), vTA[t], vTA[t + 1], cc[t], cc[t + 1]);
+}
+
+__global__ __launch_bounds__(256) void k_gaa_g(const DevScalars* sc, const double4* __restrict__ ta_coef,
+                                               double* __restrict__ vG)
+{
+  __shared__ double4 coefTA[kNB];
+  __shared__ double v[kNB], cosk[5];
+  const double bmax = 20.;
+  const double db = bmax / (kNB - 1);
+  const int t = threadIdx.x;
+  if (t < 5) cosk[t] = cos(kPi * kGLx5[t]);
+  if (t < kNB - 1) coefTA[t] = ta_coef[t];
   __syncthreads();
-  const double ssm = pow(sqrts, 2) / pow(2 * kMProt + 2.1206, 2);
-  const double csNN = 0.1 * (34.41 + 0.2720 * pow(log(ssm), 2) + 13.07 * pow(ssm, -0.4473) -
-                             7.394 * pow(ssm, -0.5486));
   const double inv_db = 1.0 / db;
   auto ta = [&](double r) { return spline_eval(coefTA, 0.0, db, inv_db, kNB, r); };
+  const double b = blockIdx.x * db;
   if (t < kNB) {
-    const double b = t * db;
-    const double integral = simpson_fn(kNB, db, [&](int is) {
-      const double s = is * db;
-      double sum_phi = 0;
-      for (int k = 0; k < 5; ++k) {
-        const double r = sqrt(b * b + s * s + 2. * b * s * cosk[k]);
-        sum_phi += 2. * kPi * kGLw5[k] * ta(r < bmax ? r : bmax);
-      }
-      return 2. * s * ta(s < bmax ? s : bmax) * sum_phi;
-    });
-    vG[t] = exp(-csNN * integral);
+    const double s = t * db;
+    double sum_phi = 0;
+    for (int k = 0; k < 5; ++k) {
+      const double r = sqrt(b * b + s * s + 2. * b * s * cosk[k]);
+      sum_phi += 2. * kPi * kGLw5[k] * ta(r < bmax ? r : bmax);
+    }
+    v[t] = 2. * s * ta(s < bmax ? s : bmax) * sum_phi;
   }
   __syncthreads();
+  if (t == 0) vG[blockIdx.x] = exp(-sc->csNN * simpson_fn(kNB, db, [&](int is) { return v[is]; }));
+}
+
+__global__ __launch_bounds__(256) void k_gaa_spline(const double* __restrict__ vG_g, double4* out_gaa)
+{
+  __shared__ double vG[kNB], cc[kNB], gam[kNB], zd[kNB];
+  const double db = 20. / (kNB - 1);
+  const int t = threadIdx.x;
+  if (t < kNB) vG[t] = vG_g[t];
+  __syncthreads();
+  auto xk = [&](int i) { return i * db; };
   if (t == 0) {
     cc[0] = 0.0;
     cc[kNB - 1] = 0.0;
     spline_window(xk, [&](int i) { return vG[i]; }, 0, kNB - 2, 0, kNB - 2, cc, gam, zd, 1);
-    sc->csNN = csNN;
   }
   __syncthreads();
   if (t < kNB - 1) out_gaa[t] = spline_coef(xk(t), xk(t + 1), vG[t], vG[t + 1], cc[t], cc[t + 1]);
@@ -255,15 +274,20 @@ __device__ const double kBkSigtn[160] = {0., .3125, .3930, .4401, .4582, .3774, 
 // First-call initialisation of the calcBreakupProb statics for Pb
 // (src/UpcCrossSection.cpp:866-969); the one-neutron tables (eee, sa) feed
 // only the unused p1n sum and are not built.
-__global__ void k_breakup_init(double g1, double* ee, double* se, DevScalars* sc)
+// The photo-absorption tables of calcBreakupProb's first call
+// (src/UpcCrossSection.cpp:866-969) for this context's gamma.  Entries are
+// independent except the Regge energies s_j = s_{j-1} * 10^0.05, which one
+// thread chains in the reference's order; the rest is filled in parallel.
+__global__ __launch_bounds__(256) void k_breakup_init(double g1, double* ee, double* se, DevScalars* sc)
 {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  __shared__ double sreg[160];
+  const int t = threadIdx.x;
   const double gammatarg = 2. * g1 * g1 - 1.;
   const double hbarcmev = 197.3269718;
   const double pi = 3.14159;
   const int zp = 82, ap = 208;
   const double si1 = 640., gw = 4.05, o1 = 13.42, o0 = 7.4;
-  for (int i = 0; i < kMaxEE; ++i) {
+  for (int i = t; i < kMaxEE; i += blockDim.x) {
     ee[i] = 0.;
     se[i] = 0.;
   }
@@ -271,36 +295,56 @@ __global__ void k_breakup_init(double g1, double* ee, double* se, DevScalars* sc
   const double scon = .1 * gw * gw * si1;
   const double zcon = zp / (gammatarg * (pi) * (hbarcmev)) * zp / (gammatarg * (pi) * (hbarcmev)) / 137.04;
   const int ne = int((25. - o0) / delo) + 1;
-  for (int i = 1; i <= ne; ++i) {
-    ee[i] = o0 + (i - 1) * delo;
-    se[i] = scon * ee[i] * ee[i] / (((o1 * o1 - ee[i] * ee[i]) * (o1 * o1 - ee[i] * ee[i])) + ee[i] * ee[i] * gw * gw);
-  }
-  int ij = ne;
-  for (int j = 1; j <= 27; ++j) { ij++; ee[ij] = kBkE3[j]; se[ij] = .1 * ap * kBkS3[j] / 208.; }
-  for (int j = 1; j <= 22; ++j) { ij++; ee[ij] = kBkE1[j]; se[ij] = .1 * ap * kBkS1[j] / 208.; }
-  for (int j = 9; j <= 70; ++j) {
-    ij++;
-    ee[ij] = ee[ij - 1] + 25.;
-    se[ij] = .1 * (zp * kBkSigt[j] + (ap - zp) * kBkSigtn[j]);
-  }
-  for (int j = 1; j <= 11; ++j) { ij++; ee[ij] = kBkE2[j]; se[ij] = .1 * ap * kBkS2[j]; }
   const double x = .0677, y = .129, eps = .0808, eta = .4525, em = .94;
-  const double exx = pow(10., .05);
-  double s = .002 * em * ee[ij];
   int ictr = 100;
   if (gammatarg > (2. * 150. * 150.)) ictr = 150;
-  for (int j = 1; j <= ictr; ++j) {
-    ij++;
-    s = s * exx;
-    ee[ij] = 1000. * .5 * (s - em * em) / em;
-    const double pom = x * pow(s, eps);
-    const double vec = y * pow(s, (-eta));
-    se[ij] = .1 * .65 * ap * (pom + vec);
+  // index layout: GDR 1..ne, then 27 + 22 + 62 + 11 tabulated, then ictr Regge, then the sentinel
+  const int i3 = ne, i1 = i3 + 27, i25 = i1 + 22, i2 = i25 + 62, ir = i2 + 11;
+  __syncthreads();  // zeroing done before the entries are written
+  for (int i = 1 + t; i <= ne; i += blockDim.x) {
+    const double e = o0 + (i - 1) * delo;
+    ee[i] = e;
+    se[i] = scon * e * e / (((o1 * o1 - e * e) * (o1 * o1 - e * e)) + e * e * gw * gw);
   }
-  ee[ij + 1] = 99999999999.;
-  sc->zcon = zcon;
-  sc->o0 = o0;
-  sc->nee = ij + 1;
+  for (int j = 1 + t; j <= 27; j += blockDim.x) {
+    ee[i3 + j] = kBkE3[j];
+    se[i3 + j] = .1 * ap * kBkS3[j] / 208.;
+  }
+  for (int j = 1 + t; j <= 22; j += blockDim.x) {
+    ee[i1 + j] = kBkE1[j];
+    se[i1 + j] = .1 * ap * kBkS1[j] / 208.;
+  }
+  // ee = previous + 25 from kBkE1[22] = 440: integers, exact in any order
+  for (int j = 9 + t; j <= 70; j += blockDim.x) {
+    ee[i25 + j - 8] = kBkE1[22] + 25. * (j - 8);
+    se[i25 + j - 8] = .1 * (zp * kBkSigt[j] + (ap - zp) * kBkSigtn[j]);
+  }
+  for (int j = 1 + t; j <= 11; j += blockDim.x) {
+    ee[i2 + j] = kBkE2[j];
+    se[i2 + j] = .1 * ap * kBkS2[j];
+  }
+  if (t == 0) {
+    const double exx = pow(10., .05);
+    double sv = .002 * em * kBkE2[11];  // ee[ir] = the last E2 energy
+    for (int j = 1; j <= ictr; ++j) {
+      sv = sv * exx;
+      sreg[j - 1] = sv;
+    }
+  }
+  __syncthreads();
+  for (int j = 1 + t; j <= ictr; j += blockDim.x) {
+    const double sv = sreg[j - 1];
+    ee[ir + j] = 1000. * .5 * (sv - em * em) / em;
+    const double pom = x * pow(sv, eps);
+    const double vec = y * pow(sv, (-eta));
+    se[ir + j] = .1 * .65 * ap * (pom + vec);
+  }
+  if (t == 0) {
+    ee[ir + ictr + 1] = 99999999999.;
+    sc->zcon = zcon;
+    sc->o0 = o0;
+    sc->nee = ir + ictr + 1;
+  }
 }
 
 // Per-b Coulomb breakup probability (src/UpcCrossSection.cpp:972-1018).
@@ -512,12 +556,15 @@ int build_tables(upc_ctx* c, hipStream_t s)
   {
   PhaseScope ph(c, s, UPC_PHASE_TABLES);
   hipLaunchKernelGGL(k_ws_rho0, dim3(1), dim3(64), 0, s, p.R, p.a, p.A, c->d_scalars);
-  hipLaunchKernelGGL(k_gaa, dim3(1), dim3(256), 0, s, p.R, p.a, p.sqrts, c->d_scalars, c->d_ta, c->d_gaa);
+  hipLaunchKernelGGL(k_gaa_ta, dim3(1), dim3(256), 0, s, p.R, p.a, p.sqrts, c->d_scalars, c->d_ta);
+  // G_AA knot values go through the head of the spline scratch (used again only later on the stream)
+  hipLaunchKernelGGL(k_gaa_g, dim3(kNB), dim3(256), 0, s, c->d_scalars, c->d_ta, c->d_spline_scratch);
+  hipLaunchKernelGGL(k_gaa_spline, dim3(1), dim3(256), 0, s, c->d_spline_scratch, c->d_gaa);
   hipLaunchKernelGGL(k_ff_values, dim3(nblk(kNQ2, 256)), dim3(256), 0, s, p.R, p.a, c->d_scalars, c->d_ff_y);
   UPC_HIP(hipGetLastError());
   int rc = solve_spline(c, kQ2min, kDQ2, c->d_ff_y, kNQ2, c->d_ff_c, c->d_ff, s);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_breakup_init, dim3(1), dim3(64), 0, s, c->g1, c->d_ee, c->d_se, c->d_scalars);
+  hipLaunchKernelGGL(k_breakup_init, dim3(1), dim3(256), 0, s, c->g1, c->d_ee, c->d_se, c->d_scalars);
   if (p.breakup_mode > 1) {
     hipLaunchKernelGGL(k_breakup_values, dim3(nblk((long)kNBkUsed * kBkLanes, 256)), dim3(256), 0, s, c->g1,
                        p.breakup_mode, c->d_ee, c->d_se, c->d_scalars, kNBkUsed, (const double*)nullptr,
