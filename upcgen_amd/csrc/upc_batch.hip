@@ -79,31 +79,30 @@ __global__ void k_batch_point_rows(const BatchJob* __restrict__ jobs, const int*
                                    long total_rows, const double* __restrict__ row_k,
                                    const double* __restrict__ row_logd, const int* __restrict__ row_cnt,
                                    const int* __restrict__ row_off, double* __restrict__ flux,
-                                   int* __restrict__ tasks)
+                                   int* __restrict__ tasks, double2* __restrict__ task_bk)
 {
   const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= total_rows * kNB1) return;
   const long r = gid / kNB1;
   const BatchJob& J = jobs[find_job(row0, njobs, r)];
-  point_row_node((int)r, (int)(gid % kNB1), J.v.fp, row_k, row_logd, row_cnt, row_off, flux, tasks);
+  point_row_node((int)r, (int)(gid % kNB1), J.v.fp, row_k, row_logd, row_cnt, row_off, flux, tasks, task_bk);
 }
 
 // QAGS task source over all jobs: task id = global row * 128 + node.
 struct BatchTask {
   const int* __restrict__ tasks;
-  const double* __restrict__ row_k;
-  const double* __restrict__ row_logd;
+  const double2* __restrict__ bk;
   const BatchJob* __restrict__ jobs;
   const int* __restrict__ row0;
   int njobs;
   __device__ void get(int t, FluxTask* ft) const
   {
     const int id = tasks[t];
+    const double2 v = bk[t];
     const int r = id >> 7, j = id & 127;
     const BatchJob& J = jobs[find_job(row0, njobs, r)];
-    double bl, bh;
-    ft->b = row_node(J.v.fp.bmin, row_logd[r], j, &bl, &bh);
-    ft->k = row_k[r];
+    ft->b = v.x;
+    ft->k = v.y;
     ft->out = (long)r * kNB1 + j;
     ft->ff = J.ff;
     ft->g1 = J.v.fp.g1;
@@ -149,6 +148,7 @@ struct upc_batch {
   long long* d_part = nullptr;
   double* d_flux = nullptr;
   int* d_tasks = nullptr;
+  double2* d_task_bk = nullptr;
   int* d_overflow = nullptr;
   unsigned long long* d_counters = nullptr;  // [0] evals [1] bad [2] overflow n [4] tasks [5] cursor
   QagsBigArrays* d_big = nullptr;
@@ -179,7 +179,7 @@ void free_batch(upc_batch* b)
   if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
   void* ptrs[] = {b->d_jobs, b->d_pt0, b->d_row0, b->d_blk0[0], b->d_blk0[1], b->d_blk0[2], b->d_pts,
                   b->d_row_k, b->d_row_logd, b->d_row_cnt, b->d_row_off, b->d_part, b->d_flux, b->d_tasks,
-                  b->d_overflow, b->d_counters, b->d_big};
+                  b->d_task_bk, b->d_overflow, b->d_counters, b->d_big};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (upc_ctx* c : b->ctx) upc_ctx_destroy(c);
@@ -200,8 +200,9 @@ int enqueue(upc_batch* b, hipStream_t s)
   hipLaunchKernelGGL(k_scan_apply<>, dim3(b->nparts), dim3(kScanBlock), 0, s, b->d_row_cnt, b->nrows, b->d_part,
                      b->d_row_off);
   hipLaunchKernelGGL(k_batch_point_rows, dim3(nblk(b->nrows * kNB1, 256)), dim3(256), 0, s, b->d_jobs, b->d_row0,
-                     nj, b->nrows, b->d_row_k, b->d_row_logd, b->d_row_cnt, b->d_row_off, b->d_flux, b->d_tasks);
-  BatchTask ts{b->d_tasks, b->d_row_k, b->d_row_logd, b->d_jobs, b->d_row0, nj};
+                     nj, b->nrows, b->d_row_k, b->d_row_logd, b->d_row_cnt, b->d_row_off, b->d_flux, b->d_tasks,
+                     b->d_task_bk);
+  BatchTask ts{b->d_tasks, b->d_task_bk, b->d_jobs, b->d_row0, nj};
   const int qblocks = b->ncu * kQagsBlockChoices[b->qags_choice];
   hipLaunchKernelGGL((k_flux_qags<kQagsCap, BatchTask>), dim3(qblocks), dim3(256), 0, s, ts, b->d_counters + 4,
                      b->d_counters + 5, b->d_flux, (int*)nullptr, b->d_counters, b->d_overflow,
@@ -277,6 +278,7 @@ int upc_batch_create(const upc_config* cfgs, int njobs, void* stream, upc_batch*
     BA(b->d_part, b->nparts);
     BA(b->d_flux, nr * kNB1);
     BA(b->d_tasks, nr * kNB1);
+    BA(b->d_task_bk, nr * kNB1);
     BA(b->d_overflow, nr * kNB1);
     BA(b->d_counters, 8);
     BA(b->d_big, b->nbig);
@@ -398,7 +400,7 @@ int upc_batch_autotune(upc_batch* b, void* stream, upc_tune* h_out)
   UPC_HIP(hipEventCreate(&e0));
   UPC_HIP(hipEventCreate(&e1));
   const int nj = (int)b->jobs.size();
-  BatchTask ts{b->d_tasks, b->d_row_k, b->d_row_logd, b->d_jobs, b->d_row0, nj};
+  BatchTask ts{b->d_tasks, b->d_task_bk, b->d_jobs, b->d_row0, nj};
   for (int i = 0; i < 3; ++i) {
     // QAGS alone: clear the counters except the task total, then time one pass
     hipLaunchKernelGGL(k_batch_zero, dim3(1), dim3(64), 0, s, b->d_counters, 4);

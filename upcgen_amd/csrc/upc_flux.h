@@ -142,14 +142,16 @@ struct FluxFormIntegrand {
 __device__ __forceinline__ void point_row_node(int r, int j, const FluxParams& fp, const double* __restrict__ row_k,
                                                const double* __restrict__ row_logd,
                                                const int* __restrict__ row_cnt, const int* __restrict__ row_off,
-                                               double* __restrict__ flux, int* __restrict__ tasks)
+                                               double* __restrict__ flux, int* __restrict__ tasks,
+                                               double2* __restrict__ task_bk)
 {
-  if (j < row_cnt[r]) {
-    tasks[row_off[r] + j] = r * 128 + j;
-    return;
-  }
   double bl, bh;
   const double b = row_node(fp.bmin, row_logd[r], j, &bl, &bh);
+  if (j < row_cnt[r]) {
+    tasks[row_off[r] + j] = r * 128 + j;
+    task_bk[row_off[r] + j] = make_double2(b, row_k[r]);
+    return;
+  }
   flux[(long)r * kNB1 + j] = flux_point(b, row_k[r], fp.g1, fp.factor);
 }
 
@@ -164,19 +166,18 @@ struct FluxTask {
 };
 
 // Task sources of the QAGS kernel: get(t) describes task t.
-struct GridTask {  // flux-row nodes of one context: id = row * 128 + j
+struct GridTask {  // flux-row nodes of one context: id = row * 128 + j, (b, k) beside it
   const int* __restrict__ tasks;
-  const double* __restrict__ row_k;
-  const double* __restrict__ row_logd;
+  const double2* __restrict__ bk;
   FluxParams fp;
   const double4* ff;
   __device__ void get(int t, FluxTask* ft) const
   {
     const int id = tasks[t];
+    const double2 v = bk[t];
     const int r = id >> 7, j = id & 127;
-    double bl, bh;
-    ft->b = row_node(fp.bmin, row_logd[r], j, &bl, &bh);
-    ft->k = row_k[r];
+    ft->b = v.x;
+    ft->k = v.y;
     ft->out = (long)r * kNB1 + j;
     ft->ff = ff;
     ft->g1 = fp.g1;

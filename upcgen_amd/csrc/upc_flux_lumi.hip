@@ -51,11 +51,12 @@ __global__ void k_rows(const double* __restrict__ pts, int npts, int nrows, Flux
 __global__ void k_flux_point_rows(int nrows, FluxParams fp, const double* __restrict__ row_k,
                                   const double* __restrict__ row_logd, const int* __restrict__ row_cnt,
                                   const int* __restrict__ row_off, double* __restrict__ flux,
-                                  int* __restrict__ tasks)
+                                  int* __restrict__ tasks, double2* __restrict__ task_bk)
 {
   const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= (long)nrows * kNB1) return;
-  point_row_node((int)(gid / kNB1), (int)(gid % kNB1), fp, row_k, row_logd, row_cnt, row_off, flux, tasks);
+  point_row_node((int)(gid / kNB1), (int)(gid % kNB1), fp, row_k, row_logd, row_cnt, row_off, flux, tasks,
+                 task_bk);
 }
 
 // ---------------------------------------------------------------------------
@@ -84,6 +85,7 @@ static int ensure_work(upc_ctx* c, size_t npts)
   UPC_HIP(hipMalloc(&w.d_row_off, sizeof(int) * rows));
   UPC_HIP(hipMalloc(&w.d_flux, sizeof(double) * rows * kNB1));
   UPC_HIP(hipMalloc(&w.d_tasks, sizeof(int) * rows * kNB1));
+  UPC_HIP(hipMalloc(&w.d_task_bk, sizeof(double2) * rows * kNB1));
   UPC_HIP(hipMalloc(&w.d_overflow, sizeof(int) * rows * kNB1));
   UPC_HIP(hipMalloc(&w.d_counters, sizeof(unsigned long long) * 8));
   UPC_HIP(hipMalloc(&w.d_points, sizeof(double) * 4 * npts));
@@ -97,7 +99,7 @@ static int ensure_work(upc_ctx* c, size_t npts)
 void free_work(upc_ctx* c)
 {
   LumiWork& w = c->work;
-  void* ptrs[] = {w.d_row_k, w.d_row_logd, w.d_row_cnt, w.d_row_off, w.d_flux, w.d_tasks,
+  void* ptrs[] = {w.d_row_k, w.d_row_logd, w.d_row_cnt, w.d_row_off, w.d_flux, w.d_tasks, w.d_task_bk,
                   w.d_overflow, w.d_counters, w.d_points, w.d_part};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -142,12 +144,12 @@ static int run_flux(upc_ctx* c, int npts, int mirror_ny, hipStream_t s)
   hipLaunchKernelGGL(k_scan_apply<>, dim3(nparts), dim3(kScanBlock), 0, s, w.d_row_cnt, (long)nrows, w.d_part,
                      w.d_row_off);
   hipLaunchKernelGGL(k_flux_point_rows, dim3(nblk((long)nrows * kNB1, 256)), dim3(256), 0, s, nrows, fp,
-                     w.d_row_k, w.d_row_logd, w.d_row_cnt, w.d_row_off, w.d_flux, w.d_tasks);
+                     w.d_row_k, w.d_row_logd, w.d_row_cnt, w.d_row_off, w.d_flux, w.d_tasks, w.d_task_bk);
   UPC_HIP(hipGetLastError());
   }
   if (!c->cfg.flux_point) {
     PhaseScope ph(c, s, UPC_PHASE_QAGS);
-    GridTask ts{w.d_tasks, w.d_row_k, w.d_row_logd, fp, c->d_ff};
+    GridTask ts{w.d_tasks, w.d_task_bk, fp, c->d_ff};
     hipLaunchKernelGGL((k_flux_qags<kQagsCap, GridTask>), dim3(qags_persistent_blocks()), dim3(256), 0, s, ts,
                        w.d_counters + 4, w.d_counters + 5, w.d_flux, (int*)nullptr, w.d_counters,
                        w.d_overflow, c->qags_cap);

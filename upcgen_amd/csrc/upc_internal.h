@@ -40,6 +40,7 @@ struct LumiWork {
   double* d_flux = nullptr;       // [rows][120]
   size_t tasks_cap = 0;
   int* d_tasks = nullptr;         // row*128 + j
+  double2* d_task_bk = nullptr;   // (b, k) of each task
   unsigned long long* d_counters = nullptr;  // [0] task cursor, [1] qags bad, [2] evals, [3] overflow n
   int* d_overflow = nullptr;      // task ids that exceeded the fast workspace
   double* d_points = nullptr;     // (M, Y, dm, dy) per point
