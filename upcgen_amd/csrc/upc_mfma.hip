@@ -303,9 +303,20 @@ __global__ __launch_bounds__(256) void k_mfma_edge(int npts, int mirror_out, con
     sw2[wv][j] = flux[(long)r2 * kNB1 + j] * b2 * (bh - bl);
   }
   __syncthreads();
-  if (lane == 0) {
-    spre[wv][0] = 0;
-    for (int j = 0; j < kNB2; ++j) spre[wv][j + 1] = spre[wv][j] + sw2[wv][j];
+  {  // prefix sums of the W2 weights: wave scan of the two halves
+    double c0 = sw2[wv][lane], c1 = lane + 64 < kNB2 ? sw2[wv][lane + 64] : 0.0;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const double u0 = __shfl_up(c0, o), u1 = __shfl_up(c1, o);
+      if (lane >= o) {
+        c0 += u0;
+        c1 += u1;
+      }
+    }
+    const double half = __shfl(c0, 63);
+    spre[wv][lane + 1] = c0;
+    if (lane + 64 < kNB2) spre[wv][lane + 65] = half + c1;
+    if (lane == 0) spre[wv][0] = 0;
   }
   __syncthreads();
   if (!live) return;
