@@ -206,7 +206,7 @@ int enqueue(upc_batch* b, hipStream_t s)
   const int qblocks = b->ncu * kQagsBlockChoices[b->qags_choice];
   hipLaunchKernelGGL((k_flux_qags<kQagsCap, BatchTask>), dim3(qblocks), dim3(256), 0, s, ts, b->d_counters + 4,
                      b->d_counters + 5, b->d_flux, (int*)nullptr, b->d_counters, b->d_overflow,
-                     kQagsCap);
+                     kQagsCap, (const double4*)nullptr);
   hipLaunchKernelGGL((k_flux_qags_big<BatchTask>), dim3(b->nbig / 64), dim3(64), 0, s, ts, b->d_overflow,
                      b->d_counters, b->d_flux, (int*)nullptr, b->d_big, b->nbig);
   const int li = b->lumi_choice, w = kLumiWaveChoices[li];
@@ -409,7 +409,7 @@ int upc_batch_autotune(upc_batch* b, void* stream, upc_tune* h_out)
     const int qblocks = b->ncu * kQagsBlockChoices[i];
     hipLaunchKernelGGL((k_flux_qags<kQagsCap, BatchTask>), dim3(qblocks), dim3(256), 0, s, ts, b->d_counters + 4,
                        b->d_counters + 5, b->d_flux, (int*)nullptr, b->d_counters, b->d_overflow,
-                     kQagsCap);
+                     kQagsCap, (const double4*)nullptr);
     UPC_HIP(hipEventRecord(e1, s));
     UPC_HIP(hipEventSynchronize(e1));
     float ms = 0;

@@ -216,11 +216,39 @@ struct PointTask {  // arbitrary (b, k) points of one context
 #define UPC_QAGS_MIN_WAVES 4
 #endif
 
+// First QK21 pass of every task (the whole range [0, 10]): one task per lane
+// at a time, every lane busy, results kept for k_flux_qags.
+template <class TS>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(UPC_QAGS_MIN_WAVES, 8))) void k_flux_first(
+    TS ts, const unsigned long long* __restrict__ ntasks_p, double4* __restrict__ first)
+{
+  __shared__ double fvbuf[20 * 256];
+  const long ntasks = (long)*ntasks_p;
+  double* fv = fvbuf + (threadIdx.x >> 6) * 20 * 64 + (threadIdx.x & 63);
+  const long stride = (long)gridDim.x * blockDim.x;
+  FluxFormIntegrand f;
+  for (long t0 = (long)blockIdx.x * blockDim.x; t0 < ntasks; t0 += stride) {
+    const long t = t0 + (threadIdx.x & ~63) + (threadIdx.x & 63);
+    const bool live = t < ntasks;
+    FluxTask ft{};
+    ts.get((int)(live ? t : ntasks - 1), &ft);
+    f.ff = ft.ff;
+    f.b = ft.b;
+    f.w2g2 = ft.k * ft.k / ft.g1 / ft.g1;
+    const QK21Out o = qk21_lds(f, kQagsA, kQagsB, fv, 64);
+    if (live) first[t] = make_double4(o.result, o.abserr, o.resabs, o.resasc);
+  }
+}
+
+// first (may be null): the tasks' first QK21 passes from k_flux_first.  With
+// it, a lane that takes a task applies that pass at once (finishing the
+// integral when it already meets the tolerance), so every step in the loop
+// below is a two-pass bisection step.
 template <int CAP, class TS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(UPC_QAGS_MIN_WAVES, 8))) void k_flux_qags(
     TS ts, const unsigned long long* __restrict__ ntasks_p, unsigned long long* __restrict__ cursor,
     double* __restrict__ flux, int* __restrict__ ier_out, unsigned long long* __restrict__ stats,
-    int* __restrict__ overflow, int fast_cap)
+    int* __restrict__ overflow, int fast_cap, const double4* __restrict__ first)
 {
   __shared__ double fvbuf[20 * 256];
   typename QagsSmallStore<CAP>::Arrays lists;
@@ -234,10 +262,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(UPC_QAGS_MI
   FluxTask ft{};
   FluxFormIntegrand f;
   unsigned long long my_evals = 0, my_bad = 0;
+  auto retire = [&](int st) {  // a finished or overflowing integral leaves the lane
+    if (st == kQagsDone) {
+      const double Q = q.result / ft.A;
+      flux[ft.out] = ft.factor * Q * Q / ft.k;
+      if (ier_out) ier_out[ft.out] = q.ier;
+      my_evals += (unsigned long long)q.neval();
+      my_bad += q.ier != 0;
+      task = -1;
+    } else if (st == kQagsOverflow) {
+      const unsigned long long slot = atomicAdd(&stats[2], 1ull);
+      overflow[slot] = (int)task;
+      task = -1;
+    }
+  };
   while (true) {
-    const bool need = task < 0;
-    const unsigned long long mask = __ballot(need);
-    if (mask) {
+    bool need = task < 0;
+    unsigned long long mask;
+    while ((mask = __ballot(need)) != 0) {
       const int leader = __ffsll((long long)mask) - 1;
       unsigned long long base = 0;
       if (lane == leader) base = atomicAdd(cursor, (unsigned long long)__popcll(mask));
@@ -252,8 +294,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(UPC_QAGS_MI
           f.b = ft.b;
           f.w2g2 = ft.k * ft.k / ft.g1 / ft.g1;
           q.start();
+          need = false;
+          if (first) {
+            const double4 v = first[t];
+            const QK21Out o{v.x, v.y, v.z, v.w};
+            const int st = q.consume(o, o);
+            retire(st);
+            need = task < 0;
+          }
         } else {
           task = LONG_MAX;
+          need = false;
         }
       }
     }
@@ -269,19 +320,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(UPC_QAGS_MI
         if (k == 0) o = r;
         else o2 = r;
       }
-      const int st = q.consume(o, o2);
-      if (st == kQagsDone) {
-        const double Q = q.result / ft.A;
-        flux[ft.out] = ft.factor * Q * Q / ft.k;
-        if (ier_out) ier_out[ft.out] = q.ier;
-        my_evals += (unsigned long long)q.neval();
-        my_bad += q.ier != 0;
-        task = -1;
-      } else if (st == kQagsOverflow) {
-        const unsigned long long slot = atomicAdd(&stats[2], 1ull);
-        overflow[slot] = (int)task;
-        task = -1;
-      }
+      retire(q.consume(o, o2));
     }
   }
   for (int o = 32; o > 0; o >>= 1) {

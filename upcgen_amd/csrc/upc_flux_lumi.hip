@@ -86,6 +86,7 @@ static int ensure_work(upc_ctx* c, size_t npts)
   UPC_HIP(hipMalloc(&w.d_flux, sizeof(double) * rows * kNB1));
   UPC_HIP(hipMalloc(&w.d_tasks, sizeof(int) * rows * kNB1));
   UPC_HIP(hipMalloc(&w.d_task_bk, sizeof(double2) * rows * kNB1));
+  UPC_HIP(hipMalloc(&w.d_first, sizeof(double4) * rows * kNB1));
   UPC_HIP(hipMalloc(&w.d_overflow, sizeof(int) * rows * kNB1));
   UPC_HIP(hipMalloc(&w.d_counters, sizeof(unsigned long long) * 8));
   UPC_HIP(hipMalloc(&w.d_points, sizeof(double) * 4 * npts));
@@ -99,7 +100,8 @@ static int ensure_work(upc_ctx* c, size_t npts)
 void free_work(upc_ctx* c)
 {
   LumiWork& w = c->work;
-  void* ptrs[] = {w.d_row_k, w.d_row_logd, w.d_row_cnt, w.d_row_off, w.d_flux, w.d_tasks, w.d_task_bk,
+  void* ptrs[] = {w.d_row_k,   w.d_row_logd, w.d_row_cnt, w.d_row_off, w.d_flux, w.d_tasks, w.d_task_bk,
+                  w.d_first,
                   w.d_overflow, w.d_counters, w.d_points, w.d_part};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -150,9 +152,11 @@ static int run_flux(upc_ctx* c, int npts, int mirror_ny, hipStream_t s)
   if (!c->cfg.flux_point) {
     PhaseScope ph(c, s, UPC_PHASE_QAGS);
     GridTask ts{w.d_tasks, w.d_task_bk, fp, c->d_ff};
+    hipLaunchKernelGGL((k_flux_first<GridTask>), dim3(qags_persistent_blocks()), dim3(256), 0, s, ts,
+                       w.d_counters + 4, w.d_first);
     hipLaunchKernelGGL((k_flux_qags<kQagsCap, GridTask>), dim3(qags_persistent_blocks()), dim3(256), 0, s, ts,
                        w.d_counters + 4, w.d_counters + 5, w.d_flux, (int*)nullptr, w.d_counters,
-                       w.d_overflow, c->qags_cap);
+                       w.d_overflow, c->qags_cap, (const double4*)w.d_first);
     QagsBigArrays* ws = nullptr;
     int rc = big_ws(256, &ws);
     if (rc) return rc;
@@ -388,9 +392,11 @@ int launch_flux_form(upc_ctx* c, const double* d_b, const double* d_k, int n, do
   hipLaunchKernelGGL(k_flux_form_classify, dim3(nblk(n, 256)), dim3(256), 0, s, d_b, d_k, n, fp, d_out, d_ier,
                      w.d_tasks, w.d_counters);
   PointTask ts{d_b, d_k, w.d_tasks, fp, c->d_ff};
+  hipLaunchKernelGGL((k_flux_first<PointTask>), dim3(qags_persistent_blocks()), dim3(256), 0, s, ts,
+                     w.d_counters + 4, w.d_first);
   hipLaunchKernelGGL((k_flux_qags<kQagsCap, PointTask>), dim3(qags_persistent_blocks()), dim3(256), 0, s, ts,
                      w.d_counters + 4, w.d_counters + 5, d_out, d_ier, w.d_counters, w.d_overflow,
-                     c->qags_cap);
+                     c->qags_cap, (const double4*)w.d_first);
   QagsBigArrays* ws = nullptr;
   rc = big_ws(256, &ws);
   if (rc) return rc;

@@ -41,6 +41,7 @@ struct LumiWork {
   size_t tasks_cap = 0;
   int* d_tasks = nullptr;         // row*128 + j
   double2* d_task_bk = nullptr;   // (b, k) of each task
+  double4* d_first = nullptr;     // first QK21 pass of each task (result, abserr, resabs, resasc)
   unsigned long long* d_counters = nullptr;  // [0] task cursor, [1] qags bad, [2] evals, [3] overflow n
   int* d_overflow = nullptr;      // task ids that exceeded the fast workspace
   double* d_points = nullptr;     // (M, Y, dm, dy) per point
