@@ -116,23 +116,23 @@ def batch_bench(seed, reps):
 
 
 def surrogate_bench(ctx, L, save_path):
-    """Train on 2^20 drawn points (512 systems x 2048; about half fall in the
-    surrogate's domain), evaluate on 2^16 held-out points (32 new systems x
-    2048), then serve this context's grid."""
+    """Train on 2^21 drawn points (1024 systems x 2048; about half fall in the
+    surrogate's domain) for 80,000 steps of 8192, evaluate on 2^16 held-out
+    points (32 new systems x 2048), then serve this context's grid."""
     import numpy as np
     import torch
     import upcgen_amd as U
     from upcgen_amd import surrogate as S
     t0 = time.perf_counter()
-    X, r, _, _ = S.make_dataset(512, 2048, seed=11)
+    X, r, _, _ = S.make_dataset(1024, 2048, seed=11)
     torch.cuda.synchronize()
     t_data = time.perf_counter() - t0
     mu, sig = float(r.mean()), float(r.std())
     s = S.Surrogate(11)
     s.norm = (mu, sig)
     t1 = time.perf_counter()
-    steps = 20000
-    loss = s.train(X.contiguous(), ((r - mu) / sig).float().contiguous(), steps=steps, batch=4096, lr=2e-3)
+    steps, batch = 80000, 8192
+    loss = s.train(X.contiguous(), ((r - mu) / sig).float().contiguous(), steps=steps, batch=batch, lr=2e-3)
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t1
     Xv, _, lgv, L0v = S.make_dataset(0, 2048, 16, systems=S.sample_systems(32, 18))
@@ -143,7 +143,7 @@ def surrogate_bench(ctx, L, save_path):
     base = (L0h + mu - lgh).abs().cpu().numpy()
     pe = (scale * spread).cpu().numpy()
     acc = pe <= 1e-3
-    res = {"train_rows": int(X.shape[0]), "heldout_rows": int(Xh.shape[0]), "steps": steps, "batch": 4096,
+    res = {"train_rows": int(X.shape[0]), "heldout_rows": int(Xh.shape[0]), "steps": steps, "batch": batch,
            "data_s": round(t_data, 2), "train_s": round(t_train, 2),
            "loss_first_last": [float(loss[:, :50].mean()), float(loss[:, -50:].mean())],
            "err_scale": round(scale, 2),
