@@ -37,10 +37,18 @@ __device__ __forceinline__ double node_b2(double b1, double b2, double c)
 
 // The phi sum of one node pair: sum_k w_k P(min(b, 20)) G(b) (G = 1 for
 // b >= 20); polarised: cos^2 and sin^2 weighted sums.
+//
+// Below the hadronic knee (b < ~11 fm for Pb-Pb) the G_AA spline holds only
+// the decaying ringing of the natural spline, |G| < 1e-30.  There the
+// breakup spline is not read and P G is taken as 0: the caller subtracts
+// p20 sum_k w_k from the same sum, and |P G| <= |G| < gskip = p20 2^-70 is
+// below 2^-70 of that term, 10^5 times below the rounding that the order of
+// the pair sums already brings.  This removes about three quarters of the
+// breakup-spline gathers, the loads that bound the contraction.
 template <bool POL>
 __device__ __forceinline__ void node_k(double b1, double b2, const double* cosk, const double* w5,
-                                       int mode, double p20, const double4* gaa, const double4* __restrict__ bk,
-                                       double* ks, double* kp)
+                                       int mode, double p20, double gskip, const double4* gaa,
+                                       const double4* __restrict__ bk, double* ks, double* kp)
 {
   const double db = 20. / (kNB - 1);
   double s = 0.0, p = 0.0;
@@ -52,8 +60,9 @@ __device__ __forceinline__ void node_k(double b1, double b2, const double* cosk,
     if (x < 400.) {  // sqrt(x) >= 20 whenever x >= 400
       const double b = sqrt(x);
       if (b < 20.) {
-        if (mode != 1) breakup = spline_eval_near(bk, kBkMin, kDBk, 1.0 / kDBk, kNBkUsed, b);
         gaa_v = spline_eval_near(gaa, 0.0, db, 1.0 / db, kNB, b);
+        if (mode != 1)
+          breakup = fabs(gaa_v) < gskip ? 0.0 : spline_eval_near(bk, kBkMin, kDBk, 1.0 / kDBk, kNBkUsed, b);
       }
     }
     if (POL) {  // w5[5 + k] = sin^2(phi_k)
@@ -569,13 +578,14 @@ __device__ __forceinline__ void lumi_fp64_body(const LumiView& v, int p0, LumiSm
     }
   }
   const int T = sm.spre[wv][kNB1];
+  const double gskip = p20 * 0x1p-70;
   double accs = 0, accp = 0;
   int i = 0;  // row of pair t: pre[i] <= t < pre[i + 1]; t only grows
   for (int t = lane; t < T; t += 64) {
     while (sm.spre[wv][i + 1] <= t) ++i;
     const int j = sm.sjlo[wv][i] + (t - sm.spre[wv][i]);
     double ks, kp;
-    node_k<POL>(sm.sb1[wv][i], sm.sb2[wv][j], sm.cosk, sm.w5, v.mode, p20, sm.gaa, v.bk, &ks, &kp);
+    node_k<POL>(sm.sb1[wv][i], sm.sb2[wv][j], sm.cosk, sm.w5, v.mode, p20, gskip, sm.gaa, v.bk, &ks, &kp);
     const double ww = sm.sw1[wv][i] * sm.sw2[wv][j];
     accs += ww * (ks - s0s);
     if (POL) accp += ww * (kp - s0p);
