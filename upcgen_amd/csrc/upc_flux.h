@@ -39,15 +39,15 @@ __device__ __forceinline__ double node_b2(double b1, double b2, double c)
 // b >= 20); polarised: cos^2 and sin^2 weighted sums.
 //
 // Below the hadronic knee (b < ~11 fm for Pb-Pb) the G_AA spline holds only
-// the decaying ringing of the natural spline, |G| < 1e-30.  There the
-// breakup spline is not read and P G is taken as 0: the caller subtracts
-// p20 sum_k w_k from the same sum, and |P G| <= |G| < gskip = p20 2^-70 is
-// below 2^-70 of that term, 10^5 times below the rounding that the order of
-// the pair sums already brings.  This removes about three quarters of the
-// breakup-spline gathers, the loads that bound the contraction.
+// the decaying ringing of the natural spline, |G| < 1e-30.  Wherever
+// |G| < gskip = p20 2^-70, P G is taken as 0 and the breakup spline is not
+// read: the caller subtracts p20 sum_k w_k from the same sum, and |P G| <=
+// |G| is below 2^-70 of that term, 10^5 times below the rounding that the
+// order of the pair sums already brings.  Below bskip (x < bskip2) the
+// bound holds for every b, so such nodes skip sqrt and G as well.
 template <bool POL>
 __device__ __forceinline__ void node_k(double b1, double b2, const double* cosk, const double* w5,
-                                       int mode, double p20, double gskip, const double4* gaa,
+                                       int mode, double p20, double gskip, double bskip2, const double4* gaa,
                                        const double4* __restrict__ bk, double* ks, double* kp)
 {
   const double db = 20. / (kNB - 1);
@@ -57,7 +57,9 @@ __device__ __forceinline__ void node_k(double b1, double b2, const double* cosk,
     const double c = cosk[k];
     const double x = node_b2(b1, b2, POL ? -c : c);
     double breakup = mode != 1 ? p20 : 1., gaa_v = 1.;
-    if (x < 400.) {  // sqrt(x) >= 20 whenever x >= 400
+    if (x < bskip2) {
+      breakup = 0.0;
+    } else if (x < 400.) {  // sqrt(x) >= 20 whenever x >= 400
       const double b = sqrt(x);
       if (b < 20.) {
         gaa_v = spline_eval_near(gaa, 0.0, db, 1.0 / db, kNB, b);
@@ -461,6 +463,7 @@ struct LumiSmem {
   double sb1[WAVES][kNB1], sb2[WAVES][kNB2];
   double sw1[WAVES][kNB1], sw2[WAVES][kNB2];
   int spre[WAVES][kNB1 + 1], sjlo[WAVES][kNB1];
+  double sp2[WAVES][kNB2 + 1];  // prefix sums of sw2
   double cosk[5], w5[10];
 };
 
@@ -502,9 +505,34 @@ __device__ __forceinline__ void lumi_fp64_body(const LumiView& v, int p0, LumiSm
     }
   }
   __syncthreads();
-  // the extreme-cos node has the smallest b for every pair
+  // the extreme-cos node has the smallest b for every pair, cmax the largest
   const double cext = POL ? -sm.cosk[0] : sm.cosk[4];
+  const double cmax = POL ? -sm.cosk[4] : sm.cosk[0];
   if (!live) return;
+  // prefix sums of sw2: sp2[j] = sum_{j' < j} sw2[j']
+  {
+    double a0 = sm.sw2[wv][lane], a1 = lane + 64 < kNB2 ? sm.sw2[wv][lane + 64] : 0.0;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const double u0 = __shfl_up(a0, o), u1 = __shfl_up(a1, o);
+      if (lane >= o) {
+        a0 += u0;
+        a1 += u1;
+      }
+    }
+    const double half = __shfl(a0, 63);
+    sm.sp2[wv][lane + 1] = a0;
+    if (lane + 64 < kNB2) sm.sp2[wv][lane + 65] = half + a1;
+    if (lane == 0) sm.sp2[wv][0] = 0.0;
+  }
+  const double bskip = v.sc->bskip, bskip2 = bskip * bskip;
+  // Pairs whose largest node is below bskip have P G = 0 at every node, so
+  // they add -s0 sw1[i] sw2[j]: per row, -s0 sw1[i] sp2[ja] for the run
+  // j < ja (b^2 at cmax > 0 grows with b2).  Those pairs are in the band
+  // (b^2 < bskip^2 <= 400 at every node), which therefore starts at j = 0;
+  // the walk takes the band from ja on.
+  double wan = 0.0;  // sum over rows of sw1[i] sp2[ja]
+  wave_sync_lds();
   // Band of row i: the j with b^2 < 400 at the extreme cosine.  b^2 is convex
   // in b2 (smallest at b2 = -b1 cext), so the band is the run of j around
   // that vertex: one search for the vertex, one for each end.
@@ -543,7 +571,17 @@ __device__ __forceinline__ void lumi_fp64_body(const LumiView& v, int p0, LumiSm
           if (inb(mid)) a = mid;
           else b = mid - 1;
         }
-        cnt[h] = a - jlo + 1;
+        int ja = 0, e = kNB2;
+        while (ja < e) {
+          const int mid = (ja + e) >> 1;
+          if (node_b2(b1, b2s[mid], cmax) < bskip2) ja = mid + 1;
+          else e = mid;
+        }
+        if (ja > 0) {
+          wan += sm.sw1[wv][i] * sm.sp2[wv][ja];
+          jlo = ja;
+        }
+        cnt[h] = a >= jlo ? a - jlo + 1 : 0;
       }
       sm.sjlo[wv][i] = jlo;
     }
@@ -585,7 +623,8 @@ __device__ __forceinline__ void lumi_fp64_body(const LumiView& v, int p0, LumiSm
     while (sm.spre[wv][i + 1] <= t) ++i;
     const int j = sm.sjlo[wv][i] + (t - sm.spre[wv][i]);
     double ks, kp;
-    node_k<POL>(sm.sb1[wv][i], sm.sb2[wv][j], sm.cosk, sm.w5, v.mode, p20, gskip, sm.gaa, v.bk, &ks, &kp);
+    node_k<POL>(sm.sb1[wv][i], sm.sb2[wv][j], sm.cosk, sm.w5, v.mode, p20, gskip, bskip2, sm.gaa, v.bk, &ks,
+                &kp);
     const double ww = sm.sw1[wv][i] * sm.sw2[wv][j];
     accs += ww * (ks - s0s);
     if (POL) accp += ww * (kp - s0p);
@@ -600,12 +639,13 @@ __device__ __forceinline__ void lumi_fp64_body(const LumiView& v, int p0, LumiSm
     accp += __shfl_down(accp, o);
     sw1t += __shfl_down(sw1t, o);
     sw2t += __shfl_down(sw2t, o);
+    wan += __shfl_down(wan, o);
   }
   if (lane == 0) {
     const double pre = 2 * kPi * kPi * M;
-    const double tot_s = s0s * (sw1t * sw2t) + accs;
+    const double tot_s = s0s * (sw1t * sw2t - wan) + accs;
     if (POL) {
-      const double tot_p = s0p * (sw1t * sw2t) + accp;
+      const double tot_p = s0p * (sw1t * sw2t - wan) + accp;
       const double ls = pre * tot_s * dm * dy, lp = pre * tot_p * dm * dy;
       v.Ls[p] = ls;
       v.Lp[p] = lp;

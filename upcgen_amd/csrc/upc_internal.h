@@ -16,6 +16,7 @@ struct DevScalars {
   double rho0;
   double csNN;
   double p20;       // breakup spline at b = 20 (1 for mode 1)
+  double bskip;     // |G_AA spline| < p20 2^-71 on [0, bskip)
   double zcon;      // calcBreakupProb statics
   double o0;
   int nee;          // index of the sentinel energy

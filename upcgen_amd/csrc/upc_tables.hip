@@ -443,10 +443,29 @@ __global__ __launch_bounds__(256) void k_breakup_values(double g1, int mode, con
   out[i] = P;
 }
 
-__global__ void k_p20(const double4* __restrict__ bk, int mode, DevScalars* sc)
+// p20 and bskip, one wave.  bskip is the knot before the first G_AA interval
+// whose bound |y| + |b| h + |c| h^2 + |d| h^3 (h a little over the knot
+// spacing, so rounding of the interval estimate is covered) reaches
+// p20 2^-71: below it every evaluation of the G_AA spline, on whichever
+// interval the estimate names, is smaller than p20 2^-70 in magnitude.
+__global__ __launch_bounds__(64) void k_p20(const double4* __restrict__ bk, const double4* __restrict__ gaa, int mode,
+                                            DevScalars* sc)
 {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  sc->p20 = mode == 1 ? 1.0 : spline_eval(bk, kBkMin, kDBk, 1.0 / kDBk, kNBkUsed, 20.);
+  const int lane = threadIdx.x;
+  const double p20 = mode == 1 ? 1.0 : spline_eval(bk, kBkMin, kDBk, 1.0 / kDBk, kNBkUsed, 20.);
+  const double db = 20. / (kNB - 1), h = db * 1.001;
+  const double lim = p20 * 0x1p-71;
+  int first = kNB - 1;
+  for (int i = lane; i < kNB - 1; i += 64) {
+    const double4 c = gaa[i];
+    const double bound = fabs(c.x) + h * (fabs(c.y) + h * (fabs(c.z) + h * fabs(c.w)));
+    if (!(bound < lim)) first = min(first, i);
+  }
+  for (int o = 32; o > 0; o >>= 1) first = min(first, __shfl_xor(first, o));
+  if (lane == 0) {
+    sc->p20 = p20;
+    sc->bskip = first > 0 ? (first - 1) * db : 0.0;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -573,7 +592,7 @@ int build_tables(upc_ctx* c, hipStream_t s)
     rc = solve_spline(c, kBkMin, kDBk, c->d_bk_y, kNBkUsed, c->d_bk_c, c->d_bk, s);
     if (rc) return rc;
   }
-  hipLaunchKernelGGL(k_p20, dim3(1), dim3(64), 0, s, c->d_bk, p.breakup_mode, c->d_scalars);
+  hipLaunchKernelGGL(k_p20, dim3(1), dim3(64), 0, s, c->d_bk, c->d_gaa, p.breakup_mode, c->d_scalars);
   UPC_HIP(hipGetLastError());
   }
   UPC_HIP(hipMemcpyAsync(&c->h_scalars, c->d_scalars, sizeof(DevScalars), hipMemcpyDeviceToHost, s));
