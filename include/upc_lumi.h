@@ -71,6 +71,8 @@ typedef struct upc_info {
   double mfma_spacing;   /* shared b-grid spacing of the MFMA forms [fm] */
   long long surrogate_points;    /* bins of the last surrogate grid served by the network */
   long long surrogate_fallback;  /* bins of the last surrogate grid recomputed by quadrature */
+  double bskip;          /* [fm]: below it |G_AA spline| < p20 2^-70, so the FP64 contraction
+                            takes P G as 0 there (no reference counterpart) */
 } upc_info;
 
 /* Phases timed with HIP events when timing is enabled (upc_ctx_set_timing). */

@@ -32,16 +32,25 @@ def test_library_exports_every_declared_symbol():
     assert sorted(U.SIGNATURES) == declared()
 
 
-def test_config_struct_matches_header():
-    import upcgen_amd as U
+def _header_fields(struct):
     text = open(HEADER).read()
-    body = text[text.index("typedef struct upc_config"):text.index("} upc_config;")]
+    body = text[text.index(f"typedef struct {struct}"):text.index(f"}} {struct};")]
     body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
-    fields = re.findall(r"(int|double)\s+([^;]+);", body)
+    fields = re.findall(r"(int|double|long long)\s+([^;]+);", body)
     names = []
     for _, decl in fields:
         names += [d.strip() for d in decl.split(",")]
-    assert names == [f for f, _ in U.Config._fields_]
+    return names
+
+
+def test_config_struct_matches_header():
+    import upcgen_amd as U
+    assert _header_fields("upc_config") == [f for f, _ in U.Config._fields_]
+
+
+def test_info_struct_matches_header():
+    import upcgen_amd as U
+    assert _header_fields("upc_info") == [f for f, _ in U.Info._fields_]
 
 
 def test_error_path_without_device():

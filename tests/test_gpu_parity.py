@@ -117,6 +117,34 @@ def test_lumi_grid_formfactor_fixtures(gpu_ctx):
 
 
 @pytest.mark.parametrize("kw", [
+    dict(PB, breakup_mode=2),
+    dict(PB, breakup_mode=1),
+    dict(PB, breakup_mode=4),
+    dict(Z=54, A=129, R=5.4889, a=0.54, sqrts=5440.0, breakup_mode=3),
+    dict(Z=79, A=197, R=6.3691, a=0.54, sqrts=200.0, breakup_mode=2),
+    dict(Z=8, A=16, R=2.4809, a=0.54, sqrts=7000.0, breakup_mode=2),
+])
+def test_contraction_skip_radius(gpu_ctx, kw):
+    """The FP64 contraction takes P G as 0 below info.bskip, where the G_AA
+    spline must be below p20 2^-70 in magnitude on every interval the
+    estimate can name (knots approached from both sides included); and the
+    radius must not be vacuous (at least half of 2R, where the hadronic
+    overlap shuts G off)."""
+    g = gpu_ctx(flux_point=0, nm=3, ny=3, **kw)
+    inf = g.info
+    bs, p20 = inf.bskip, inf.p20
+    print(f"bskip {bs:.4f} fm, p20 {p20:.4g}")
+    assert 0.5 * 2 * kw["R"] <= bs < 20.0
+    db = 20.0 / 199
+    knots = np.arange(int(bs / db) + 1) * db
+    b = np.concatenate([np.linspace(0.0, bs, 40001, endpoint=False), knots * (1 - 4e-16), knots,
+                        knots * (1 + 4e-16)])
+    b = b[(b >= 0) & (b < bs)]
+    G = _np(g.nuclear_overlap(_t(b), 0))
+    assert np.max(np.abs(G)) < p20 * 2.0 ** -70
+
+
+@pytest.mark.parametrize("kw", [
     dict(flux_point=1, breakup_mode=4, use_polarized=0),
     dict(flux_point=0, breakup_mode=1, use_polarized=1),
     dict(flux_point=1, breakup_mode=2, use_polarized=1),

@@ -59,7 +59,7 @@ class Info(C.Structure):
                 ("mNucl", C.c_double), ("rho0", C.c_double), ("csNN", C.c_double),
                 ("p20", C.c_double), ("qags_calls", C.c_longlong), ("qags_bad", C.c_longlong),
                 ("qags_evals", C.c_longlong), ("mfma_ksteps", C.c_longlong), ("mfma_spacing", C.c_double),
-                ("surrogate_points", C.c_longlong), ("surrogate_fallback", C.c_longlong)]
+                ("surrogate_points", C.c_longlong), ("surrogate_fallback", C.c_longlong), ("bskip", C.c_double)]
 
 
 class EventOpts(C.Structure):

@@ -144,6 +144,7 @@ int upc_ctx_info(upc_ctx* c, upc_info* h)
   h->rho0 = c->h_scalars.rho0;
   h->csNN = c->h_scalars.csNN;
   h->p20 = c->h_scalars.p20;
+  h->bskip = c->h_scalars.bskip;
   h->qags_calls = c->last_qags_calls;
   h->qags_bad = c->last_qags_bad;
   h->qags_evals = c->last_qags_evals;
