@@ -30,6 +30,25 @@ __device__ __forceinline__ double row_node(double bmin, double logd, int j, doub
   return (*bh + *bl) / 2.;
 }
 
+// Position guess on a row's log-spaced nodes b_j = (bmin e^{j logd} +
+// bmin e^{(j+1) logd}) / 2 = c0 e^{j logd}: the first j with b_j >= v, from
+// one float log2 (right up to +-1 where v lies within rounding of a node).
+// Callers step from it to the exact index with their own comparisons.
+struct LogGrid {
+  float l2c0, s;  // log2(c0), ln 2 / logd
+  __device__ __forceinline__ void init(double bmin, double logd)
+  {
+    l2c0 = (float)log2(bmin * (1.0 + exp(logd)) * 0.5);
+    s = (float)(0.69314718055994531 / logd);
+  }
+  __device__ __forceinline__ int first_ge(double v, int n) const
+  {
+    if (!(v > 0.0)) return 0;
+    const float g = ceilf((__log2f((float)v) - l2c0) * s);
+    return g < 0.f ? 0 : (g > (float)n ? n : (int)g);
+  }
+};
+
 __device__ __forceinline__ double node_b2(double b1, double b2, double c)
 {
   return fma(2.0 * b1 * b2, c, fma(b1, b1, b2 * b2));
@@ -486,12 +505,14 @@ __device__ __forceinline__ void lumi_fp64_body(const LumiView& v, int p0, LumiSm
   const FluxParams& fp = v.fp;
   const double p20 = v.sc->p20;
   double M = 0, dm = 1, dy = 1;
+  LogGrid lg2{};
   if (live) {
     M = v.pts[4 * p];
     dm = v.pts[4 * p + 2];
     dy = v.pts[4 * p + 3];
     const int r2 = k2_row(p, v.npts, fp.mirror_ny);
     const double logd1 = v.row_logd[p], logd2 = v.row_logd[r2];
+    lg2.init(fp.bmin, logd2);
     const double* f1 = v.flux + (long)p * kNB1;
     const double* f2 = v.flux + (long)r2 * kNB1;
     for (int j = lane; j < kNB1; j += 64) {
@@ -535,7 +556,9 @@ __device__ __forceinline__ void lumi_fp64_body(const LumiView& v, int p0, LumiSm
   wave_sync_lds();
   // Band of row i: the j with b^2 < 400 at the extreme cosine.  b^2 is convex
   // in b2 (smallest at b2 = -b1 cext), so the band is the run of j around
-  // that vertex: one search for the vertex, one for each end.
+  // that vertex.  The vertex and both ends (roots b2 = vtx -+ sqrt(400 -
+  // b1^2 (1 - cext^2))) are guessed on the log grid of b2 and stepped to
+  // exactness with the same comparisons a bisection would make.
   int cnt[2];
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
@@ -546,37 +569,30 @@ __device__ __forceinline__ void lumi_fp64_body(const LumiView& v, int p0, LumiSm
       const double* b2s = sm.sb2[wv];
       auto inb = [&](int j) { return node_b2(b1, b2s[j], cext) < 400.0; };
       const double vtx = -b1 * cext;
-      int lo = 0, hi = kNB2;
-      while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (b2s[mid] < vtx) lo = mid + 1;
-        else hi = mid;
-      }
+      int lo = lg2.first_ge(vtx, kNB2);  // first j with b2_j >= vtx
+      while (lo > 0 && b2s[lo - 1] >= vtx) --lo;
+      while (lo < kNB2 && b2s[lo] < vtx) ++lo;
       int jc = -1;
       if (lo < kNB2 && inb(lo)) jc = lo;
       if (lo > 0 && inb(lo - 1)) jc = lo - 1;
       int jlo = 0;
       if (jc >= 0) {
-        int a = 0, b = jc;  // first j in band: outside before it, inside from it to jc
-        while (a < b) {
-          const int mid = (a + b) >> 1;
-          if (inb(mid)) b = mid;
-          else a = mid + 1;
-        }
+        const double d = 400.0 - b1 * b1 * (1.0 - cext * cext), sd = sqrt(d > 0.0 ? d : 0.0);
+        int a = lg2.first_ge(vtx - sd, kNB2);  // first j in band: outside before it, inside from it to jc
+        a = a > jc ? jc : a;
+        while (a > 0 && inb(a - 1)) --a;
+        while (a < jc && !inb(a)) ++a;
         jlo = a;
-        a = jc;
-        b = kNB2 - 1;  // last j in band
-        while (a < b) {
-          const int mid = (a + b + 1) >> 1;
-          if (inb(mid)) a = mid;
-          else b = mid - 1;
-        }
-        int ja = 0, e = kNB2;
-        while (ja < e) {
-          const int mid = (ja + e) >> 1;
-          if (node_b2(b1, b2s[mid], cmax) < bskip2) ja = mid + 1;
-          else e = mid;
-        }
+        a = lg2.first_ge(vtx + sd, kNB2) - 1;  // last j in band
+        a = a < jc ? jc : (a > kNB2 - 1 ? kNB2 - 1 : a);
+        while (a < kNB2 - 1 && inb(a + 1)) ++a;
+        while (a > jc && !inb(a)) --a;
+        // first j with b^2 >= bskip^2 at cmax (b^2 grows with b2 there)
+        auto low = [&](int j) { return node_b2(b1, b2s[j], cmax) < bskip2; };
+        const double da = bskip2 - b1 * b1 * (1.0 - cmax * cmax);
+        int ja = da > 0.0 ? lg2.first_ge(-b1 * cmax + sqrt(da), kNB2) : 0;
+        while (ja > 0 && !low(ja - 1)) --ja;
+        while (ja < kNB2 && low(ja)) ++ja;
         if (ja > 0) {
           wan += sm.sw1[wv][i] * sm.sp2[wv][ja];
           jlo = ja;

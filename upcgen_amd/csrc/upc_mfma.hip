@@ -19,15 +19,16 @@
 // At b = 20 fm g is not smooth: P is clamped there (slope jump) and G_AA
 // switches from its spline to exactly 1 (value jump J = g(20) - g(20-)).
 // Interpolating that edge costs ~1e-5 of L at h = 0.2 fm, so it is split off:
-//   g = g_c + e,   e(b) = [b < 20] (-J + kappa(b)),
-//   kappa(b) = -g'(20-) (20 - b) (1 - x^2)^2,  x = (20 - b) / delta <= 1,
-// which leaves g_c continuous with a continuous slope at 20 fm.  Only D_c
-// (from g_c) is interpolated; the edge part
+//   g = g_c + e,   e(b) = [b < 20] (-J + s (400 - b^2)),  s = -g'(20-) / 40,
+// which leaves g_c continuous with a continuous slope at 20 fm (e jumps by J
+// and has slope -40 s = g'(20-) just below 20).  Only D_c (from g_c) is
+// interpolated; the ramp is a quadratic in (b1, b2), which the 4-point
+// Lagrange interpolation reproduces exactly.  The edge part
 //   sum_ij W1_i W2_j sum_k w_k e(b_k(i, j))
 // is evaluated exactly at the rows' own node pairs (kernel k_mfma_edge): for
 // fixed (i, k) the j with b_k < 20 form one contiguous range (b_k^2 is a
-// parabola in b2), so the J term is a difference of prefix sums of W2 and
-// kappa is summed over the few nodes of that range with b_k > 20 - delta.
+// parabola in b2), and with b_k^2 = b1^2 + b2^2 + 2 b1 b2 c the sum over
+// that range is a difference of prefix sums of W2, W2 b2 and W2 b2^2.
 // D_c is symmetric, vanishes for |B_u - B_v| >= 20 fm (every node is then at or
 // beyond 20 fm, where K = S0 in the same arithmetic) and for B >= 20 /
 // sqrt(1 - c^2) with c the extreme node cosine, whatever the partner.  So the
@@ -49,19 +50,12 @@ typedef float f32x8 __attribute__((ext_vector_type(8)));
 typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
 
 constexpr int kMfmaWaves = 4;      // waves per block in the contraction kernel
-constexpr double kEdgeWidth = 2.0;  // delta: support of the slope-matching ramp below 20 fm
 
-// e(b) of the edge split; J = edge[0], -g'(20-) = edge[1].
-__device__ __forceinline__ double edge_e(double b, double J, double s)
+// e(b) of the edge split below b = 20 (x = b^2 < 400); J = edge[0],
+// -g'(20-) = edge[1], so s = -g'(20-) / 40 = edge[1] / 40.
+__device__ __forceinline__ double edge_e(double x, double J, double sl)
 {
-  if (b >= 20.) return 0.;
-  double e = -J;
-  const double x = (20. - b) * (1.0 / kEdgeWidth);
-  if (x < 1.) {
-    const double t = 1. - x * x;
-    e += s * (20. - b) * t * t;
-  }
-  return e;
+  return -J + sl * (1.0 / 40.0) * (400.0 - x);
 }
 
 // J and -g'(20-) of g(b) = P(b) G(b) (P = 1 in mode 1) from the splines,
@@ -181,11 +175,12 @@ __global__ __launch_bounds__(256) void k_mfma_kernel_matrix(MfmaGrid g, int mode
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
       const double c = cosk[k];
-      const double b = sqrt(node_b2(b1, b2, POL ? -c : c));
+      const double x = node_b2(b1, b2, POL ? -c : c);
+      const double b = sqrt(x);
       double gc = br;
       if (b < 20.) {
         const double breakup = mode != 1 ? spline_eval(bk, kBkMin, kDBk, 1.0 / kDBk, kNBkUsed, b) : 1.;
-        gc = breakup * spline_eval(gaa, 0.0, db, 1.0 / db, kNB, b) - edge_e(b, J, sl);
+        gc = breakup * spline_eval(gaa, 0.0, db, 1.0 / db, kNB, b) - edge_e(x, J, sl);
       }
       if (POL) {
         ks += gc * w5[k] * c * c;
@@ -217,7 +212,10 @@ __global__ __launch_bounds__(256) void k_mfma_project(int nrows, const double* _
   const bool live = r0 < nrows;
   const int r = live ? r0 : nrows - 1;
   const double logd = row_logd[r];
+  LogGrid lg;
+  lg.init(fp.bmin, logd);
   double tot = 0;
+  int ulim = -1;  // last column any node feeds: all later columns are 0
   for (int j = lane; j < kNB1; j += 64) {
     double bl, bh;
     const double b = row_node(fp.bmin, logd, j, &bl, &bh);
@@ -236,25 +234,29 @@ __global__ __launch_bounds__(256) void k_mfma_project(int nrows, const double* _
       sL[wv][j][3] = x * (x - 1.) * (x - 2.) / 6.;
     }
     sI[wv][j] = i0;
+    ulim = i0 == INT_MAX ? ulim : max(ulim, i0 + 3);
   }
-  for (int o = 32; o > 0; o >>= 1) tot += __shfl_xor(tot, o);
+  for (int o = 32; o > 0; o >>= 1) {
+    tot += __shfl_xor(tot, o);
+    ulim = max(ulim, __shfl_xor(ulim, o));
+  }
   __syncthreads();
   // gather: node i feeds columns i0_i .. i0_i + 3; i0 is nondecreasing in i.
   // Two passes over the columns: the row maximum first, then the scaled store.
   auto column = [&](int u) {
-    int lo = 0, hi = kNB1;  // first i with i0_i >= u - 3
-    while (lo < hi) {
-      const int mid = (lo + hi) >> 1;
-      if (sI[wv][mid] < u - 3) lo = mid + 1;
-      else hi = mid;
-    }
+    // first i with i0_i >= u - 3: i0_i >= u - 3 when b_i >= (u - 2) h (the
+    // clamps of i0 aside), guessed on the log grid and stepped to exactness
+    int lo = u <= 3 ? 0 : lg.first_ge((u - 2) * g.h, kNB1);
+    while (lo > 0 && sI[wv][lo - 1] >= u - 3) --lo;
+    while (lo < kNB1 && sI[wv][lo] < u - 3) ++lo;
     double a = 0;
     for (int i = lo; i < kNB1 && sI[wv][i] <= u; ++i) a += sW[wv][i] * sL[wv][i][u - sI[wv][i]];
     return a;
   };
   double amax = 0;
   int last = -1;
-  for (int u = lane; u < g.nu; u += 64) {
+  const int uend = min(g.nu, ulim + 1);
+  for (int u = lane; u < uend; u += 64) {
     const double a = column(u);
     amax = fmax(amax, fabs(a));
     if (a != 0) last = u;
@@ -266,7 +268,7 @@ __global__ __launch_bounds__(256) void k_mfma_project(int nrows, const double* _
   if (!live) return;
   const double inv = amax > 0 ? 1.0 / amax : 0.0;
   for (int u = lane; u < g.nu_pad; u += 64)
-    split_store(u < g.nu ? column(u) * inv : 0.0, Wh, Wl, W32, (long)r * g.nu_pad + u);
+    split_store(u < uend ? column(u) * inv : 0.0, Wh, Wl, W32, (long)r * g.nu_pad + u);
   if (lane == 0) {
     wscale[r] = amax;
     wsum[r] = tot;
@@ -284,7 +286,8 @@ __global__ __launch_bounds__(256) void k_mfma_edge(int npts, int mirror_out, con
                                                    const double* __restrict__ flux, FluxParams fp,
                                                    const double* __restrict__ edge, double* __restrict__ out)
 {
-  __shared__ double sb1[4][kNB1], sb2[4][kNB2], sw1[4][kNB1], sw2[4][kNB2], spre[4][kNB2 + 1];
+  // spre[.][m][j] = sum_{j' < j} W2_j' b2_j'^m, m = 0, 1, 2
+  __shared__ double sb1[4][kNB1], sb2[4][kNB2], sw1[4][kNB1], spre[4][3][kNB2 + 1];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int nwork = lumi_work_items(npts, fp.mirror_ny, mirror_out);
   const int q0 = blockIdx.x * 4 + wv;
@@ -293,18 +296,26 @@ __global__ __launch_bounds__(256) void k_mfma_edge(int npts, int mirror_out, con
   work_point(live ? q0 : nwork - 1, fp.mirror_ny, mirror_out, &p, &pm);
   const int r2 = k2_row(p, npts, fp.mirror_ny);
   const double logd1 = row_logd[p], logd2 = row_logd[r2];
-  for (int j = lane; j < kNB1; j += 64) {
+  LogGrid lg2;
+  lg2.init(fp.bmin, logd2);
+  double w2[2] = {0.0, 0.0}, b2v[2] = {0.0, 0.0};  // nodes lane and lane + 64 of row r2
+  for (int h = 0; h < 2; ++h) {
+    const int j = lane + 64 * h;
+    if (j >= kNB1) break;
     double bl, bh;
     const double b1 = row_node(fp.bmin, logd1, j, &bl, &bh);
     sb1[wv][j] = b1;
     sw1[wv][j] = flux[(long)p * kNB1 + j] * b1 * (bh - bl);
     const double b2 = row_node(fp.bmin, logd2, j, &bl, &bh);
     sb2[wv][j] = b2;
-    sw2[wv][j] = flux[(long)r2 * kNB1 + j] * b2 * (bh - bl);
+    b2v[h] = b2;
+    w2[h] = flux[(long)r2 * kNB1 + j] * b2 * (bh - bl);
   }
-  __syncthreads();
-  {  // prefix sums of the W2 weights: wave scan of the two halves
-    double c0 = sw2[wv][lane], c1 = lane + 64 < kNB2 ? sw2[wv][lane + 64] : 0.0;
+  // prefix sums of W2, W2 b2, W2 b2^2: wave scans of the two halves
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    double c0 = w2[0] * (m == 0 ? 1.0 : (m == 1 ? b2v[0] : b2v[0] * b2v[0]));
+    double c1 = w2[1] * (m == 0 ? 1.0 : (m == 1 ? b2v[1] : b2v[1] * b2v[1]));
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
       const double u0 = __shfl_up(c0, o), u1 = __shfl_up(c1, o);
@@ -314,15 +325,13 @@ __global__ __launch_bounds__(256) void k_mfma_edge(int npts, int mirror_out, con
       }
     }
     const double half = __shfl(c0, 63);
-    spre[wv][lane + 1] = c0;
-    if (lane + 64 < kNB2) spre[wv][lane + 65] = half + c1;
-    if (lane == 0) spre[wv][0] = 0;
+    spre[wv][m][lane + 1] = c0;
+    if (lane + 64 < kNB2) spre[wv][m][lane + 65] = half + c1;
+    if (lane == 0) spre[wv][m][0] = 0;
   }
   __syncthreads();
   if (!live) return;
-  const double J = edge[0], sl = edge[1];
-  // e = -J exactly below b = 20 - delta: the ring split may use b^2 directly
-  constexpr double kCore2 = (20. - kEdgeWidth) * (20. - kEdgeWidth);
+  const double J = edge[0], s = edge[1] * (1.0 / 40.0);  // e = -J + s (400 - b^2) below 20
   double cosk[5];
 #pragma unroll
   for (int k = 0; k < 5; ++k) cosk[k] = edge[2 + k];
@@ -341,49 +350,46 @@ __global__ __launch_bounds__(256) void k_mfma_edge(int npts, int mirror_out, con
     // inside, and only x within rounding of 400 needs the square root
     auto inside = [&](double x) { return x < 400. && (x < 399.99999 || sqrt(x) < 20.); };
     // b_k^2 is smallest at b2 = -b1 c: the last node below it or the first above
+    // (first j with b2_j >= vtx: a log-grid guess stepped to exactness)
     const double vtx = -b1 * c;
-    int lo = 0, hi = kNB2;
-    while (lo < hi) {
-      const int mid = (lo + hi) >> 1;
-      if (sb2[wv][mid] < vtx) lo = mid + 1;
-      else hi = mid;
-    }
+    int lo = lg2.first_ge(vtx, kNB2);
+    while (lo > 0 && sb2[wv][lo - 1] >= vtx) --lo;
+    while (lo < kNB2 && sb2[wv][lo] < vtx) ++lo;
     int jc = -1;
     const double xlo = lo < kNB2 ? bk2(lo) : 1e300, xlm = lo > 0 ? bk2(lo - 1) : 1e300;
     if (inside(xlo)) jc = lo;
     if (inside(xlm) && (jc < 0 || xlm < xlo)) jc = lo - 1;
     if (jc < 0) continue;
+    // The pass region of b_k^2 < X around jc ends near the roots
+    // b2 = -b1 c -+ sqrt(X - b1^2 (1 - c^2)): each end is guessed on the log
+    // grid and stepped with pass() itself (false ... true on [0, jc], true ...
+    // false on [jc, n)), so the ends are those a bisection would find.
+    auto root = [&](double X, double sgn) {
+      const double d = X - b1 * b1 * (1.0 - c * c);
+      return vtx + sgn * sqrt(d > 0.0 ? d : 0.0);
+    };
     // first node in [0, jc] that passes (b_k falls towards jc)
-    auto first_in = [&](auto pass) {
-      int a = 0, b = jc;
-      while (a < b) {
-        const int mid = (a + b) >> 1;
-        if (pass(bk2(mid))) b = mid;
-        else a = mid + 1;
-      }
+    auto first_in = [&](auto pass, double X) {
+      int a = lg2.first_ge(root(X, -1.0), kNB2);
+      a = a > jc ? jc : a;
+      while (a > 0 && pass(bk2(a - 1))) --a;
+      while (a < jc && !pass(bk2(a))) ++a;
       return a;
     };
     // last node in [jc, n) that passes (b_k rises after jc)
-    auto last_in = [&](auto pass) {
-      int a = jc, b = kNB2 - 1;
-      while (a < b) {
-        const int mid = (a + b + 1) >> 1;
-        if (pass(bk2(mid))) a = mid;
-        else b = mid - 1;
-      }
+    auto last_in = [&](auto pass, double X) {
+      int a = lg2.first_ge(root(X, 1.0), kNB2) - 1;
+      a = a < jc ? jc : (a > kNB2 - 1 ? kNB2 - 1 : a);
+      while (a < kNB2 - 1 && pass(bk2(a + 1))) ++a;
+      while (a > jc && !pass(bk2(a))) --a;
       return a;
     };
-    auto core = [&](double x) { return x < kCore2; };
-    const int jlo = first_in(inside), jhi = last_in(inside);
-    double sum = -J * (spre[wv][jhi + 1] - spre[wv][jlo]);
-    const int r0 = jlo, r1 = jhi + 1;  // ring: [r0, l1) and [l2, r1), where 20 - delta <= b_k < 20
-    int l1 = r1, l2 = r1;
-    if (core(bk2(jc))) {
-      l1 = first_in(core);
-      l2 = last_in(core) + 1;
-    }
-    for (int j = r0; j < l1; ++j) sum += sw2[wv][j] * (edge_e(sqrt(bk2(j)), J, sl) + J);
-    for (int j = l2; j < r1; ++j) sum += sw2[wv][j] * (edge_e(sqrt(bk2(j)), J, sl) + J);
+    const int jlo = first_in(inside, 400.0), jhi = last_in(inside, 400.0);
+    // sum_{j in [jlo, jhi]} W2_j (-J + s (400 - b1^2 - b2_j^2 - 2 b1 b2_j c))
+    const double m0 = spre[wv][0][jhi + 1] - spre[wv][0][jlo];
+    const double m1 = spre[wv][1][jhi + 1] - spre[wv][1][jlo];
+    const double m2 = spre[wv][2][jhi + 1] - spre[wv][2][jlo];
+    const double sum = (-J + s * (400.0 - b1 * b1)) * m0 - s * (m2 + 2.0 * b1 * c * m1);
     const double w = sw1[wv][i] * kGLw5[k] * sum;
     if (POL) {
       acc_s += w * cs * cs;
