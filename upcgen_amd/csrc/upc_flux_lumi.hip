@@ -91,6 +91,7 @@ static int ensure_work(upc_ctx* c, size_t npts)
   UPC_HIP(hipMalloc(&w.d_counters, sizeof(unsigned long long) * 8));
   UPC_HIP(hipMalloc(&w.d_points, sizeof(double) * 4 * npts));
   UPC_HIP(hipMalloc(&w.d_part, sizeof(long long) * ((rows + 1023) / 1024 + 1)));
+  UPC_HIP(hipMalloc(&w.d_big, sizeof(QagsBigArrays) * kQagsBigLanes));
   w.rows_cap = rows;
   w.tasks_cap = rows * kNB1;
   w.points_cap = npts;
@@ -102,24 +103,10 @@ void free_work(upc_ctx* c)
   LumiWork& w = c->work;
   void* ptrs[] = {w.d_row_k,   w.d_row_logd, w.d_row_cnt, w.d_row_off, w.d_flux, w.d_tasks, w.d_task_bk,
                   w.d_first,
-                  w.d_overflow, w.d_counters, w.d_points, w.d_part};
+                  w.d_overflow, w.d_counters, w.d_points, w.d_part, w.d_big};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   w = LumiWork{};
-}
-
-static QagsBigArrays* g_big_ws = nullptr;  // lazily allocated, process lifetime
-static int g_big_ws_n = 0;
-
-static int big_ws(int n, QagsBigArrays** out)
-{
-  if (g_big_ws_n < n) {
-    if (g_big_ws) (void)hipFree(g_big_ws);
-    UPC_HIP(hipMalloc(&g_big_ws, sizeof(QagsBigArrays) * n));
-    g_big_ws_n = n;
-  }
-  *out = g_big_ws;
-  return 0;
 }
 
 static int grid_mirror_ny(const upc_ctx* c, bool grid)
@@ -157,11 +144,8 @@ static int run_flux(upc_ctx* c, int npts, int mirror_ny, hipStream_t s)
     hipLaunchKernelGGL((k_flux_qags<kQagsCap, GridTask>), dim3(qags_persistent_blocks()), dim3(256), 0, s, ts,
                        w.d_counters + 4, w.d_counters + 5, w.d_flux, (int*)nullptr, w.d_counters,
                        w.d_overflow, c->qags_cap, (const double4*)w.d_first);
-    QagsBigArrays* ws = nullptr;
-    int rc = big_ws(256, &ws);
-    if (rc) return rc;
-    hipLaunchKernelGGL((k_flux_qags_big<GridTask>), dim3(4), dim3(64), 0, s, ts, w.d_overflow, w.d_counters,
-                       w.d_flux, (int*)nullptr, ws, 256);
+    hipLaunchKernelGGL((k_flux_qags_big<GridTask>), dim3(kQagsBigLanes / 64), dim3(64), 0, s, ts, w.d_overflow,
+                       w.d_counters, w.d_flux, (int*)nullptr, (QagsBigArrays*)w.d_big, kQagsBigLanes);
     UPC_HIP(hipGetLastError());
   }
   return 0;
@@ -397,11 +381,8 @@ int launch_flux_form(upc_ctx* c, const double* d_b, const double* d_k, int n, do
   hipLaunchKernelGGL((k_flux_qags<kQagsCap, PointTask>), dim3(qags_persistent_blocks()), dim3(256), 0, s, ts,
                      w.d_counters + 4, w.d_counters + 5, d_out, d_ier, w.d_counters, w.d_overflow,
                      c->qags_cap, (const double4*)w.d_first);
-  QagsBigArrays* ws = nullptr;
-  rc = big_ws(256, &ws);
-  if (rc) return rc;
-  hipLaunchKernelGGL((k_flux_qags_big<PointTask>), dim3(4), dim3(64), 0, s, ts, w.d_overflow, w.d_counters,
-                     d_out, d_ier, ws, 256);
+  hipLaunchKernelGGL((k_flux_qags_big<PointTask>), dim3(kQagsBigLanes / 64), dim3(64), 0, s, ts, w.d_overflow,
+                     w.d_counters, d_out, d_ier, (QagsBigArrays*)w.d_big, kQagsBigLanes);
   UPC_HIP(hipGetLastError());
   return read_stats(c, s);
 }

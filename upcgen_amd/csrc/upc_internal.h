@@ -48,7 +48,10 @@ struct LumiWork {
   double* d_points = nullptr;     // (M, Y, dm, dy) per point
   long long* d_part = nullptr;    // per-block partial sums of the row-count scan
   size_t points_cap = 0;
+  void* d_big = nullptr;          // QAGS overflow workspace (kQagsBigLanes lanes)
 };
+
+constexpr int kQagsBigLanes = 256;
 
 // upc_events.hip
 int build_pt_tables(upc_ctx* c, double gtot, hipStream_t s);
