@@ -168,6 +168,41 @@ def test_lumi_grid_modes(gpu_ctx, oracle_ctx, kw):
         _lumi_check(_np(out), ref)
 
 
+@pytest.mark.parametrize("nm,ny,pol", [(3, 4, 0), (2, 2, 1), (3, 1, 0), (1, 6, 1)])
+def test_lumi_grid_even_and_single(gpu, gpu_ctx, oracle_ctx, nm, ny, pol):
+    """Grids where a Y bin is its own mirror (even ny: iy = ny / 2 shares its
+    k2 row with itself), a single Y bin and a single M bin: the FP64 form
+    against the oracle at 1e-9, the MFMA forms at 1e-5."""
+    kw = dict(flux_point=0, breakup_mode=2, use_polarized=pol, nm=nm, ny=ny)
+    g = gpu_ctx(**kw)
+    o = oracle_ctx(**kw)
+    ref, nbad = o.lumi_grid()
+    assert nbad == 0
+    refs = ref if pol else (ref,)
+    out = g.lumi_grid()
+    for a, r in zip(out if pol else (out,), refs):
+        _lumi_check(_np(a), r)
+    for form in (gpu.FORM_MFMA, gpu.FORM_MFMA_FP32):
+        m = g.lumi_grid(form)
+        for a, r in zip(m if pol else (m,), refs):
+            assert _lumi_check(_np(a), r, tol=1e-5) <= 1e-5
+
+
+def test_zero_length_point_calls(gpu_ctx):
+    """n = 0 is a valid call for every per-point entry point: nothing is
+    launched and nothing fails."""
+    g = gpu_ctx(flux_point=0, breakup_mode=2, nm=3, ny=3)
+    e = _t(np.zeros(0))
+    assert g.lumi_points(e, e).numel() == 0
+    f, ier = g.flux_form(e, e)
+    assert f.numel() == 0 and ier.numel() == 0
+    assert g.flux_point(e, e).numel() == 0
+    assert g.photon_flux(e, e).numel() == 0
+    assert g.nuclear_overlap(e).numel() == 0
+    assert g.breakup_prob(e).numel() == 0
+    assert g.form_factor(e).numel() == 0
+
+
 def test_lumi_points_arbitrary(gpu_ctx, oracle_ctx):
     g = gpu_ctx(flux_point=0, breakup_mode=2, nm=3, ny=3)
     o = oracle_ctx(flux_point=0, breakup_mode=2, nm=3, ny=3)
