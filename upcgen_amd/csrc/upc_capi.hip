@@ -64,6 +64,46 @@ static int validate(const upc_config* g)
   return 0;
 }
 
+static int alloc_host_copies(upc_ctx* c)
+{
+  UPC_HIP(hipHostMalloc((void**)&c->hp_scalars, sizeof(DevScalars), hipHostMallocDefault));
+  UPC_HIP(hipHostMalloc((void**)&c->hp_stats, sizeof(unsigned long long) * 8, hipHostMallocDefault));
+  UPC_HIP(hipEventCreateWithFlags(&c->ev_scalars, hipEventDisableTiming));
+  UPC_HIP(hipEventCreateWithFlags(&c->ev_stats, hipEventDisableTiming));
+  return 0;
+}
+
+static void free_host_copies(upc_ctx* c)
+{
+  for (hipEvent_t e : {c->ev_scalars, c->ev_stats})
+    if (e) {
+      (void)hipEventSynchronize(e);
+      (void)hipEventDestroy(e);
+    }
+  if (c->hp_scalars) (void)hipHostFree(c->hp_scalars);
+  if (c->hp_stats) (void)hipHostFree(c->hp_stats);
+  c->ev_scalars = c->ev_stats = nullptr;
+  c->hp_scalars = nullptr;
+  c->hp_stats = nullptr;
+}
+
+int host_copies(upc_ctx* c)
+{
+  if (c->scalars_pending) {
+    UPC_HIP(hipEventSynchronize(c->ev_scalars));
+    c->h_scalars = *c->hp_scalars;
+    c->scalars_pending = false;
+  }
+  if (c->stats_pending) {
+    UPC_HIP(hipEventSynchronize(c->ev_stats));
+    c->last_qags_evals = (long long)c->hp_stats[0];
+    c->last_qags_bad = (long long)c->hp_stats[1];
+    c->last_qags_calls = (long long)c->hp_stats[4];
+    c->stats_pending = false;
+  }
+  return 0;
+}
+
 }  // namespace upc
 
 using namespace upc;
@@ -102,6 +142,7 @@ int upc_ctx_create(const upc_config* cfg, void* stream, upc_ctx** out)
   c->factor = cfg->Z * cfg->Z * kAlpha / M_PI / M_PI / kHc / kHc;
   c->mNucl = (cfg->Z * kMProt + (cfg->A - cfg->Z) * kMNeut) / cfg->A;
   rc = alloc_tables(c);
+  if (!rc) rc = alloc_host_copies(c);
   if (!rc) rc = build_tables(c, S(stream));
   if (rc) {
     upc_ctx_destroy(c);
@@ -125,6 +166,7 @@ int upc_ctx_destroy(upc_ctx* ctx)
     (void)hipEventDestroy(r.beg);
     (void)hipEventDestroy(r.end);
   }
+  free_host_copies(ctx);
   free_work(ctx);
   free_tables(ctx);
   free_mfma(ctx);
@@ -137,6 +179,7 @@ int upc_ctx_destroy(upc_ctx* ctx)
 int upc_ctx_info(upc_ctx* c, upc_info* h)
 {
   if (!c || !h) return set_error("null pointer"), UPC_EINVAL;
+  if (int rc = host_copies(c)) return rc;
   h->g1 = c->g1;
   h->g2 = c->g2;
   h->factor = c->factor;

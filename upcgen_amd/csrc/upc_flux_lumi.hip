@@ -153,12 +153,9 @@ static int run_flux(upc_ctx* c, int npts, int mirror_ny, hipStream_t s)
 
 static int read_stats(upc_ctx* c, hipStream_t s)
 {
-  unsigned long long h[8];
-  UPC_HIP(hipMemcpyAsync(h, c->work.d_counters, sizeof(h), hipMemcpyDeviceToHost, s));
-  UPC_HIP(hipStreamSynchronize(s));
-  c->last_qags_evals = (long long)h[0];
-  c->last_qags_bad = (long long)h[1];
-  c->last_qags_calls = (long long)h[4];
+  UPC_HIP(hipMemcpyAsync(c->hp_stats, c->work.d_counters, sizeof(unsigned long long) * 8, hipMemcpyDeviceToHost, s));
+  UPC_HIP(hipEventRecord(c->ev_stats, s));
+  c->stats_pending = true;
   return 0;
 }
 

@@ -90,8 +90,15 @@ struct upc_ctx {
   long long t_n[UPC_NPHASES] = {};
   int device = 0;
   double g1 = 0, g2 = 0, factor = 0, mNucl = 0;
-  upc::DevScalars h_scalars{};      // host copy after init
+  upc::DevScalars h_scalars{};      // host copy after init (see host_copies())
   upc::DevScalars* d_scalars = nullptr;
+  // Device-to-host copies that only upc_ctx_info reads are queued into pinned
+  // memory behind an event instead of synchronising the stream; host_copies()
+  // waits for them when info is asked for.
+  upc::DevScalars* hp_scalars = nullptr;
+  unsigned long long* hp_stats = nullptr;  // the QAGS counter block of the last flux launch
+  hipEvent_t ev_scalars = nullptr, ev_stats = nullptr;
+  bool scalars_pending = false, stats_pending = false;
   double4* d_ta = nullptr;          // 199 intervals
   double4* d_gaa = nullptr;         // 199 intervals
   double* d_ff_y = nullptr;         // 10^6 knot values
@@ -116,6 +123,7 @@ struct upc_ctx {
 namespace upc {
 
 void set_error(const std::string& msg);
+int host_copies(upc_ctx* c);  // completes the queued device-to-host copies
 
 // Records an event pair around a phase on `s` when timing is enabled.
 struct PhaseScope {

@@ -595,8 +595,9 @@ int build_tables(upc_ctx* c, hipStream_t s)
   hipLaunchKernelGGL(k_p20, dim3(1), dim3(64), 0, s, c->d_bk, c->d_gaa, p.breakup_mode, c->d_scalars);
   UPC_HIP(hipGetLastError());
   }
-  UPC_HIP(hipMemcpyAsync(&c->h_scalars, c->d_scalars, sizeof(DevScalars), hipMemcpyDeviceToHost, s));
-  UPC_HIP(hipStreamSynchronize(s));
+  UPC_HIP(hipMemcpyAsync(c->hp_scalars, c->d_scalars, sizeof(DevScalars), hipMemcpyDeviceToHost, s));
+  UPC_HIP(hipEventRecord(c->ev_scalars, s));
+  c->scalars_pending = true;
   return 0;
 }
 
