@@ -109,8 +109,15 @@ struct upc_ctx {
   double4* d_bk = nullptr;          // kNBkUsed - 1 intervals
   double* d_ee = nullptr;           // calcBreakupProb tables ee[], se[]
   double* d_se = nullptr;
-  double* d_spline_scratch = nullptr;  // windowed tridiagonal solver scratch
+  double* d_spline_scratch = nullptr;  // windowed tridiagonal solver scratch (form-factor spline)
   size_t spline_scratch_elems = 0;
+  double* d_bk_scratch = nullptr;      // the same for the breakup spline, built concurrently
+  size_t bk_scratch_elems = 0;
+  double* d_gaa_scratch = nullptr;     // G_AA knot values
+  // The three table chains (G_AA, form factor, breakup) are independent: two
+  // side streams (default priority) run two of them beside the caller's.
+  hipStream_t side[2] = {nullptr, nullptr};
+  hipEvent_t ev_fork = nullptr, ev_join[2] = {nullptr, nullptr};
   upc::LumiWork work;
   long long last_qags_calls = 0, last_qags_bad = 0, last_qags_evals = 0;
   int qags_cap = upc::kQagsCap;  // intervals of the fast QAGS path before the 1000-interval kernel

@@ -539,31 +539,47 @@ int alloc_tables(upc_ctx* c)
   UPC_HIP(hipMalloc(&c->d_bk, sizeof(double4) * kNBkUsed));
   UPC_HIP(hipMalloc(&c->d_ee, sizeof(double) * kMaxEE));
   UPC_HIP(hipMalloc(&c->d_se, sizeof(double) * kMaxEE));
-  const int nthr = nblk(kNQ2, kSplineChunk);
-  c->spline_scratch_elems = (size_t)2 * (kSplineChunk + 2 * kSplineGuard) * nthr;
+  c->spline_scratch_elems = (size_t)2 * (kSplineChunk + 2 * kSplineGuard) * nblk(kNQ2, kSplineChunk);
   UPC_HIP(hipMalloc(&c->d_spline_scratch, sizeof(double) * c->spline_scratch_elems));
+  c->bk_scratch_elems = (size_t)2 * (kSplineChunk + 2 * kSplineGuard) * nblk(kNBkUsed, kSplineChunk);
+  UPC_HIP(hipMalloc(&c->d_bk_scratch, sizeof(double) * c->bk_scratch_elems));
+  UPC_HIP(hipMalloc(&c->d_gaa_scratch, sizeof(double) * kNB));
+  for (int i = 0; i < 2; ++i) {
+    UPC_HIP(hipStreamCreateWithFlags(&c->side[i], hipStreamNonBlocking));
+    UPC_HIP(hipEventCreateWithFlags(&c->ev_join[i], hipEventDisableTiming));
+  }
+  UPC_HIP(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
   return 0;
 }
 
 void free_tables(upc_ctx* c)
 {
+  for (int i = 0; i < 2; ++i) {
+    if (c->side[i]) {
+      (void)hipStreamSynchronize(c->side[i]);
+      (void)hipStreamDestroy(c->side[i]);
+    }
+    if (c->ev_join[i]) (void)hipEventDestroy(c->ev_join[i]);
+  }
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
   void* ptrs[] = {c->d_scalars, c->d_ta, c->d_gaa, c->d_ff_y, c->d_ff_c, c->d_ff,
-                  c->d_bk_y, c->d_bk_c, c->d_bk, c->d_ee, c->d_se, c->d_spline_scratch};
+                  c->d_bk_y, c->d_bk_c, c->d_bk, c->d_ee, c->d_se, c->d_spline_scratch,
+                  c->d_bk_scratch, c->d_gaa_scratch};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
 }
 
-static int solve_spline(upc_ctx* c, double x0, double dx, const double* y, int n, double* cc,
-                        double4* coef, hipStream_t s)
+static int solve_spline(double x0, double dx, const double* y, int n, double* cc, double4* coef,
+                        double* scratch, size_t scratch_elems, hipStream_t s)
 {
   const int m = n - 2;
   const int nthr = nblk(m, kSplineChunk);
-  if ((size_t)2 * (kSplineChunk + 2 * kSplineGuard) * nthr > c->spline_scratch_elems) {
+  if ((size_t)2 * (kSplineChunk + 2 * kSplineGuard) * nthr > scratch_elems) {
     set_error("spline scratch too small");
     return UPC_EINVAL;
   }
-  hipLaunchKernelGGL(k_spline_window, dim3(nblk(nthr, 64)), dim3(64), 0, s, x0, dx, y, n, cc,
-                     c->d_spline_scratch, kSplineChunk, kSplineGuard, nthr);
+  hipLaunchKernelGGL(k_spline_window, dim3(nblk(nthr, 64)), dim3(64), 0, s, x0, dx, y, n, cc, scratch,
+                     kSplineChunk, kSplineGuard, nthr);
   hipLaunchKernelGGL(k_spline_coefs, dim3(nblk(n - 1, 256)), dim3(256), 0, s, x0, dx, y, cc, n, coef);
   UPC_HIP(hipGetLastError());
   return 0;
@@ -574,24 +590,39 @@ int build_tables(upc_ctx* c, hipStream_t s)
   const upc_config& p = c->cfg;
   {
   PhaseScope ph(c, s, UPC_PHASE_TABLES);
+  // rho0 feeds T_A and the form factor; then three independent chains, each
+  // with its own scratch and its own fields of DevScalars: G_AA (side[0]),
+  // breakup (side[1]) and the form factor (s), joined before k_p20, which
+  // reads both splines.
   hipLaunchKernelGGL(k_ws_rho0, dim3(1), dim3(64), 0, s, p.R, p.a, p.A, c->d_scalars);
-  hipLaunchKernelGGL(k_gaa_ta, dim3(1), dim3(256), 0, s, p.R, p.a, p.sqrts, c->d_scalars, c->d_ta);
-  // G_AA knot values go through the head of the spline scratch (used again only later on the stream)
-  hipLaunchKernelGGL(k_gaa_g, dim3(kNB), dim3(256), 0, s, c->d_scalars, c->d_ta, c->d_spline_scratch);
-  hipLaunchKernelGGL(k_gaa_spline, dim3(1), dim3(256), 0, s, c->d_spline_scratch, c->d_gaa);
-  hipLaunchKernelGGL(k_ff_values, dim3(nblk(kNQ2, 256)), dim3(256), 0, s, p.R, p.a, c->d_scalars, c->d_ff_y);
+  UPC_HIP(hipEventRecord(c->ev_fork, s));
+  const hipStream_t sg = c->side[0], sb = c->side[1];
+  UPC_HIP(hipStreamWaitEvent(sg, c->ev_fork, 0));
+  UPC_HIP(hipStreamWaitEvent(sb, c->ev_fork, 0));
+  hipLaunchKernelGGL(k_gaa_ta, dim3(1), dim3(256), 0, sg, p.R, p.a, p.sqrts, c->d_scalars, c->d_ta);
+  hipLaunchKernelGGL(k_gaa_g, dim3(kNB), dim3(256), 0, sg, c->d_scalars, c->d_ta, c->d_gaa_scratch);
+  hipLaunchKernelGGL(k_gaa_spline, dim3(1), dim3(256), 0, sg, c->d_gaa_scratch, c->d_gaa);
   UPC_HIP(hipGetLastError());
-  int rc = solve_spline(c, kQ2min, kDQ2, c->d_ff_y, kNQ2, c->d_ff_c, c->d_ff, s);
-  if (rc) return rc;
-  hipLaunchKernelGGL(k_breakup_init, dim3(1), dim3(256), 0, s, c->g1, c->d_ee, c->d_se, c->d_scalars);
+  UPC_HIP(hipEventRecord(c->ev_join[0], sg));
+  hipLaunchKernelGGL(k_breakup_init, dim3(1), dim3(256), 0, sb, c->g1, c->d_ee, c->d_se, c->d_scalars);
+  int rc = 0;
   if (p.breakup_mode > 1) {
-    hipLaunchKernelGGL(k_breakup_values, dim3(nblk((long)kNBkUsed * kBkLanes, 256)), dim3(256), 0, s, c->g1,
+    hipLaunchKernelGGL(k_breakup_values, dim3(nblk((long)kNBkUsed * kBkLanes, 256)), dim3(256), 0, sb, c->g1,
                        p.breakup_mode, c->d_ee, c->d_se, c->d_scalars, kNBkUsed, (const double*)nullptr,
                        c->d_bk_y);
     UPC_HIP(hipGetLastError());
-    rc = solve_spline(c, kBkMin, kDBk, c->d_bk_y, kNBkUsed, c->d_bk_c, c->d_bk, s);
-    if (rc) return rc;
+    rc = solve_spline(kBkMin, kDBk, c->d_bk_y, kNBkUsed, c->d_bk_c, c->d_bk, c->d_bk_scratch,
+                      c->bk_scratch_elems, sb);
   }
+  UPC_HIP(hipEventRecord(c->ev_join[1], sb));
+  hipLaunchKernelGGL(k_ff_values, dim3(nblk(kNQ2, 256)), dim3(256), 0, s, p.R, p.a, c->d_scalars, c->d_ff_y);
+  UPC_HIP(hipGetLastError());
+  if (!rc)
+    rc = solve_spline(kQ2min, kDQ2, c->d_ff_y, kNQ2, c->d_ff_c, c->d_ff, c->d_spline_scratch,
+                      c->spline_scratch_elems, s);
+  UPC_HIP(hipStreamWaitEvent(s, c->ev_join[0], 0));
+  UPC_HIP(hipStreamWaitEvent(s, c->ev_join[1], 0));
+  if (rc) return rc;
   hipLaunchKernelGGL(k_p20, dim3(1), dim3(64), 0, s, c->d_bk, c->d_gaa, p.breakup_mode, c->d_scalars);
   UPC_HIP(hipGetLastError());
   }
