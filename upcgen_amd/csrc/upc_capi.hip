@@ -135,7 +135,10 @@ int upc_ctx_create(const upc_config* cfg, void* stream, upc_ctx** out)
   upc_ctx* c = new (std::nothrow) upc_ctx();
   if (!c) return set_error("host allocation failed"), UPC_ENOMEM;
   c->cfg = *cfg;
-  UPC_HIP(hipGetDevice(&c->device));
+  if (hipGetDevice(&c->device) != hipSuccess) {
+    delete c;
+    return set_error("hipGetDevice failed"), UPC_EDEVICE;
+  }
   // src/UpcGenerator.cpp:188-192 and src/UpcCrossSection.cpp:120-121
   c->g1 = cfg->sqrts / (2. * kMProt);
   c->g2 = cfg->sqrts / (2. * kMProt);
