@@ -149,6 +149,7 @@ struct upc_batch {
   double* d_flux = nullptr;
   int* d_tasks = nullptr;
   double2* d_task_bk = nullptr;
+  double4* d_first = nullptr;  // first QK21 pass of each task (k_flux_first)
   int* d_overflow = nullptr;
   unsigned long long* d_counters = nullptr;  // [0] evals [1] bad [2] overflow n [4] tasks [5] cursor
   QagsBigArrays* d_big = nullptr;
@@ -179,7 +180,7 @@ void free_batch(upc_batch* b)
   if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
   void* ptrs[] = {b->d_jobs, b->d_pt0, b->d_row0, b->d_blk0[0], b->d_blk0[1], b->d_blk0[2], b->d_pts,
                   b->d_row_k, b->d_row_logd, b->d_row_cnt, b->d_row_off, b->d_part, b->d_flux, b->d_tasks,
-                  b->d_task_bk, b->d_overflow, b->d_counters, b->d_big};
+                  b->d_task_bk, b->d_first, b->d_overflow, b->d_counters, b->d_big};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (upc_ctx* c : b->ctx) upc_ctx_destroy(c);
@@ -204,9 +205,11 @@ int enqueue(upc_batch* b, hipStream_t s)
                      b->d_task_bk);
   BatchTask ts{b->d_tasks, b->d_task_bk, b->d_jobs, b->d_row0, nj};
   const int qblocks = b->ncu * kQagsBlockChoices[b->qags_choice];
+  hipLaunchKernelGGL((k_flux_first<BatchTask>), dim3(b->ncu * kQagsBlockChoices[2]), dim3(256), 0, s, ts,
+                     b->d_counters + 4, b->d_first);
   hipLaunchKernelGGL((k_flux_qags<kQagsCap, BatchTask>), dim3(qblocks), dim3(256), 0, s, ts, b->d_counters + 4,
                      b->d_counters + 5, b->d_flux, (int*)nullptr, b->d_counters, b->d_overflow,
-                     kQagsCap, (const double4*)nullptr);
+                     kQagsCap, (const double4*)b->d_first);
   hipLaunchKernelGGL((k_flux_qags_big<BatchTask>), dim3(b->nbig / 64), dim3(64), 0, s, ts, b->d_overflow,
                      b->d_counters, b->d_flux, (int*)nullptr, b->d_big, b->nbig);
   const int li = b->lumi_choice, w = kLumiWaveChoices[li];
@@ -279,6 +282,7 @@ int upc_batch_create(const upc_config* cfgs, int njobs, void* stream, upc_batch*
     BA(b->d_flux, nr * kNB1);
     BA(b->d_tasks, nr * kNB1);
     BA(b->d_task_bk, nr * kNB1);
+    BA(b->d_first, nr * kNB1);
     BA(b->d_overflow, nr * kNB1);
     BA(b->d_counters, 8);
     BA(b->d_big, b->nbig);
@@ -402,14 +406,15 @@ int upc_batch_autotune(upc_batch* b, void* stream, upc_tune* h_out)
   const int nj = (int)b->jobs.size();
   BatchTask ts{b->d_tasks, b->d_task_bk, b->d_jobs, b->d_row0, nj};
   for (int i = 0; i < 3; ++i) {
-    // QAGS alone: clear the counters except the task total, then time one pass
+    // QAGS alone (the first passes of the warm run stay valid: they depend on
+    // the task list only): clear the counters except the task total, then time
     hipLaunchKernelGGL(k_batch_zero, dim3(1), dim3(64), 0, s, b->d_counters, 4);
     hipLaunchKernelGGL(k_batch_zero, dim3(1), dim3(64), 0, s, b->d_counters + 5, 1);
     UPC_HIP(hipEventRecord(e0, s));
     const int qblocks = b->ncu * kQagsBlockChoices[i];
     hipLaunchKernelGGL((k_flux_qags<kQagsCap, BatchTask>), dim3(qblocks), dim3(256), 0, s, ts, b->d_counters + 4,
                        b->d_counters + 5, b->d_flux, (int*)nullptr, b->d_counters, b->d_overflow,
-                     kQagsCap, (const double4*)nullptr);
+                     kQagsCap, (const double4*)b->d_first);
     UPC_HIP(hipEventRecord(e1, s));
     UPC_HIP(hipEventSynchronize(e1));
     float ms = 0;
