@@ -15,6 +15,7 @@ Prints one JSON line.  `--gpus` above 1 is not supported (one GPU per box).
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -272,17 +273,30 @@ def main():
     else:
         # MFMA FLOP issued per launch: tile steps x 16x16x32 x products per step
         # (split-bf16: hi*hi + hi*lo + lo*hi; FP32: one) for each luminosity (1 or 2)
+        # The banded GEMM does 4e10 MFMA FLOP against ~0.6 GB of operands, so
+        # it is bound by HBM: its algorithmic bytes are every projected flux
+        # row read once (4 bytes a column: bf16 hi + lo, or FP32) and the band
+        # of D; the MFMA rate is reported beside them.
         split = form == U.FORM_MFMA
         nprod = 3 if split else 1
         flop = inf.mfma_ksteps * KSTEP_FLOP * nprod
         peak = BF16_PEAK_TFLOPS if split else FP32_MFMA_PEAK_TFLOPS
         tflops = flop / (lumi_ms * 1e-3) / 1e12 if lumi_ms > 0 else 0.0
-        roof = {"bound": "mfma", "kernel": f"k_mfma_lumi ({'split-bf16' if split else 'fp32'} banded GEMM)",
-                "achieved": round(tflops, 3), "peak": peak, "unit": "TFLOP/s",
-                "frac": round(tflops / peak, 5),
-                "traffic": TRAFFIC_BYTES["k_mfma_lumi_split"] if split else None, "mfma_flop_per_launch": flop,
-                "ksteps_per_launch": inf.mfma_ksteps, "avg_launch_ms": round(lumi_ms, 4),
-                "shared_grid_spacing_fm": inf.mfma_spacing}
+        h = inf.mfma_spacing
+        cext = -math.cos(math.pi * 0.9739065285171717)  # the unpolarised grid's extreme node cosine
+        nu = int(math.floor(20.0 / math.sqrt(1.0 - cext * cext) * (1.0 + 1e-12) / h)) + 4
+        nu_pad = (nu + 31) // 32 * 32
+        band = int(math.ceil(20.0 / h)) + 1
+        rows = POINTS + NM  # k1 rows of every bin + k2 rows of the iy = 0 bins (mirror sharing)
+        alg_bytes = rows * nu_pad * 4 + nu_pad * (2 * band - 1) * 4
+        gbs = alg_bytes / (lumi_ms * 1e-3) / 1e9 if lumi_ms > 0 else 0.0
+        roof = {"bound": "hbm", "kernel": f"k_mfma_lumi ({'split-bf16' if split else 'fp32'} banded GEMM)",
+                "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(gbs / HBM_PEAK_GBS, 5), "algorithmic_bytes_per_launch": alg_bytes,
+                "traffic": TRAFFIC_BYTES["k_mfma_lumi_split"] if split else None,
+                "mfma_tflops": round(tflops, 3), "mfma_peak_tflops": peak, "mfma_frac": round(tflops / peak, 5),
+                "mfma_flop_per_launch": flop, "ksteps_per_launch": inf.mfma_ksteps,
+                "avg_launch_ms": round(lumi_ms, 4), "shared_grid_spacing_fm": h}
     compare = None
     if not args.no_compare:
         compare = {}

@@ -472,50 +472,80 @@ __global__ __launch_bounds__(64 * kMfmaWaves) void k_mfma_lumi(
   unsigned long long nk = 0;
   const int ntiles = (uend + 15) >> 4;
   const int kbmax = (kend + 31) >> 5;
-  for (int t = 0; t < ntiles; ++t) {
-    const int u0 = t * 16;
-    const int kb0 = max(0, (u0 - g.band + 1) >> 5);
-    const int kb1 = min(kbmax, (u0 + 15 + g.band - 1) / 32 + 1);
-    f32x4 acc = {0, 0, 0, 0}, accp = {0, 0, 0, 0};
-    const long rowB = (long)(u0 + m) * g.nu_pad;
-    for (int kb = kb0; kb < kb1; ++kb) {
+  // Column tile t (columns 16 t .. 16 t + 15) needs the k blocks
+  // [kb0(t), kb1(t)) of the band.  kTileGroup consecutive tiles share one
+  // load of each A fragment (the wave's 16 W2 rows at that k block), which
+  // neighbouring tiles would otherwise each re-read; every tile still
+  // accumulates its own k blocks in ascending order.  Polarised waves carry
+  // two accumulators per tile, so they group half as many tiles.
+  constexpr int kTileGroup = POL ? 2 : 4;
+  auto kb0 = [&](int t) { return max(0, (t * 16 - g.band + 1) >> 5); };
+  auto kb1 = [&](int t) { return min(kbmax, (t * 16 + 15 + g.band - 1) / 32 + 1); };
+  for (int t0 = 0; t0 < ntiles; t0 += kTileGroup) {
+    f32x4 acc[kTileGroup], accp[kTileGroup];
+#pragma unroll
+    for (int gi = 0; gi < kTileGroup; ++gi) {
+      acc[gi] = f32x4{0, 0, 0, 0};
+      accp[gi] = f32x4{0, 0, 0, 0};
+    }
+    const int tl = min(ntiles, t0 + kTileGroup) - 1;
+    for (int kb = kb0(t0); kb < kb1(tl); ++kb) {
       const int k = kb * 32 + g4 * 8;
       if (PREC == UPC_FORM_MFMA) {
         const bf16x8 ah = __builtin_bit_cast(bf16x8, *(const u16x8*)(Wh + rowA + k));
         const bf16x8 al = __builtin_bit_cast(bf16x8, *(const u16x8*)(Wl + rowA + k));
-        const bf16x8 bh = __builtin_bit_cast(bf16x8, *(const u16x8*)(Dh + rowB + k));
-        const bf16x8 bl = __builtin_bit_cast(bf16x8, *(const u16x8*)(Dl + rowB + k));
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc, 0, 0, 0);
-        if (POL) {
-          const bf16x8 ph = __builtin_bit_cast(bf16x8, *(const u16x8*)(Dh + nn + rowB + k));
-          const bf16x8 pl = __builtin_bit_cast(bf16x8, *(const u16x8*)(Dl + nn + rowB + k));
-          accp = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, ph, accp, 0, 0, 0);
-          accp = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, pl, accp, 0, 0, 0);
-          accp = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, ph, accp, 0, 0, 0);
+#pragma unroll
+        for (int gi = 0; gi < kTileGroup; ++gi) {
+          const int t = t0 + gi;
+          if (t > tl || kb < kb0(t) || kb >= kb1(t)) continue;  // wave-uniform
+          const long rowB = (long)(t * 16 + m) * g.nu_pad;
+          const bf16x8 bh = __builtin_bit_cast(bf16x8, *(const u16x8*)(Dh + rowB + k));
+          const bf16x8 bl = __builtin_bit_cast(bf16x8, *(const u16x8*)(Dl + rowB + k));
+          acc[gi] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc[gi], 0, 0, 0);
+          acc[gi] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc[gi], 0, 0, 0);
+          acc[gi] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc[gi], 0, 0, 0);
+          if (POL) {
+            const bf16x8 ph = __builtin_bit_cast(bf16x8, *(const u16x8*)(Dh + nn + rowB + k));
+            const bf16x8 pl = __builtin_bit_cast(bf16x8, *(const u16x8*)(Dl + nn + rowB + k));
+            accp[gi] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, ph, accp[gi], 0, 0, 0);
+            accp[gi] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, pl, accp[gi], 0, 0, 0);
+            accp[gi] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, ph, accp[gi], 0, 0, 0);
+          }
         }
       } else {
         const f32x8 a = *(const f32x8*)(W32 + rowA + k);
-        const f32x8 b = *(const f32x8*)(D32 + rowB + k);
 #pragma unroll
-        for (int e = 0; e < 8; ++e) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[e], b[e], acc, 0, 0, 0);
-        if (POL) {
-          const f32x8 bp = *(const f32x8*)(D32 + nn + rowB + k);
+        for (int gi = 0; gi < kTileGroup; ++gi) {
+          const int t = t0 + gi;
+          if (t > tl || kb < kb0(t) || kb >= kb1(t)) continue;  // wave-uniform
+          const long rowB = (long)(t * 16 + m) * g.nu_pad;
+          const f32x8 b = *(const f32x8*)(D32 + rowB + k);
 #pragma unroll
-          for (int e = 0; e < 8; ++e) accp = __builtin_amdgcn_mfma_f32_16x16x4f32(a[e], bp[e], accp, 0, 0, 0);
+          for (int e = 0; e < 8; ++e)
+            acc[gi] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[e], b[e], acc[gi], 0, 0, 0);
+          if (POL) {
+            const f32x8 bp = *(const f32x8*)(D32 + nn + rowB + k);
+#pragma unroll
+            for (int e = 0; e < 8; ++e)
+              accp[gi] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[e], bp[e], accp[gi], 0, 0, 0);
+          }
         }
       }
     }
-    nk += kb1 > kb0 ? kb1 - kb0 : 0;
-    const int u = u0 + m;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      float w1;
-      if (PREC == UPC_FORM_MFMA) w1 = from_bf16(Wh[rowC[i] + u]) + from_bf16(Wl[rowC[i] + u]);
-      else w1 = W32[rowC[i] + u];
-      dot_s[i] += (double)acc[i] * (double)w1;
-      if (POL) dot_p[i] += (double)accp[i] * (double)w1;
+    for (int gi = 0; gi < kTileGroup; ++gi) {
+      const int t = t0 + gi;
+      if (t > tl) continue;
+      nk += kb1(t) > kb0(t) ? kb1(t) - kb0(t) : 0;
+      const int u = t * 16 + m;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        float w1;
+        if (PREC == UPC_FORM_MFMA) w1 = from_bf16(Wh[rowC[i] + u]) + from_bf16(Wl[rowC[i] + u]);
+        else w1 = W32[rowC[i] + u];
+        dot_s[i] += (double)acc[gi][i] * (double)w1;
+        if (POL) dot_p[i] += (double)accp[gi][i] * (double)w1;
+      }
     }
   }
 #pragma unroll
