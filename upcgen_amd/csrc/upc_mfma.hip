@@ -35,8 +35,8 @@
 // contraction is a banded GEMM  Y = What2 D  over all points at once, fused
 // with the dot  Y . What1.  It runs on MFMA as split-bf16 (x = hi + lo,
 // hi*hi + hi*lo + lo*hi, FP32 accumulation) or as FP32 MFMA; the dot and the
-// S0 term are FP64.  Each What row is scaled by its max |value| (kept in FP64)
-// so the FP32 range never limits a row.
+// S0 term are FP64.  Each What row is scaled by a power of two (kept in FP64)
+// that brings its values below 1, so the FP32 range never limits a row.
 #include "upc_flux.h"
 
 #include <cmath>
@@ -214,13 +214,14 @@ __global__ __launch_bounds__(256) void k_mfma_project(int nrows, const double* _
   const double logd = row_logd[r];
   LogGrid lg;
   lg.init(fp.bmin, logd);
-  double tot = 0;
+  double tot = 0, tabs = 0;
   int ulim = -1;  // last column any node feeds: all later columns are 0
   for (int j = lane; j < kNB1; j += 64) {
     double bl, bh;
     const double b = row_node(fp.bmin, logd, j, &bl, &bh);
     const double w = flux[(long)r * kNB1 + j] * b * (bh - bl);
     tot += w;
+    tabs += fabs(w);
     sW[wv][j] = w;
     int i0 = INT_MAX;
     if (b < g.bstar) {
@@ -238,11 +239,11 @@ __global__ __launch_bounds__(256) void k_mfma_project(int nrows, const double* _
   }
   for (int o = 32; o > 0; o >>= 1) {
     tot += __shfl_xor(tot, o);
+    tabs += __shfl_xor(tabs, o);
     ulim = max(ulim, __shfl_xor(ulim, o));
   }
   __syncthreads();
   // gather: node i feeds columns i0_i .. i0_i + 3; i0 is nondecreasing in i.
-  // Two passes over the columns: the row maximum first, then the scaled store.
   auto column = [&](int u) {
     // first i with i0_i >= u - 3: i0_i >= u - 3 when b_i >= (u - 2) h (the
     // clamps of i0 aside), guessed on the log grid and stepped to exactness
@@ -253,24 +254,23 @@ __global__ __launch_bounds__(256) void k_mfma_project(int nrows, const double* _
     for (int i = lo; i < kNB1 && sI[wv][i] <= u; ++i) a += sW[wv][i] * sL[wv][i][u - sI[wv][i]];
     return a;
   };
-  double amax = 0;
-  int last = -1;
+  // One pass: each column is scaled by a power of two 2^-e with 4 sum|W| <
+  // 2^e, so every stored value is below 1 in magnitude (|column u| <= sum_i
+  // |W_i| |l_i(u)|, and the 4-point weights stay below 4 even where i0 is
+  // clamped at the grid ends) and the FP32 range never limits a row.  The
+  // scaling is exact, so the bf16 / FP32 split keeps the row's own mantissas.
+  const int e = tabs > 0 ? ilogb(4.0 * tabs) + 1 : 0;
+  const double sc = ldexp(1.0, -e);
   const int uend = min(g.nu, ulim + 1);
-  for (int u = lane; u < uend; u += 64) {
-    const double a = column(u);
-    amax = fmax(amax, fabs(a));
+  int last = -1;
+  for (int u = lane; u < g.nu_pad; u += 64) {
+    const double a = u < uend ? column(u) : 0.0;
     if (a != 0) last = u;
+    if (live) split_store(a * sc, Wh, Wl, W32, (long)r * g.nu_pad + u);
   }
-  for (int o = 32; o > 0; o >>= 1) {
-    amax = fmax(amax, __shfl_xor(amax, o));
-    last = max(last, __shfl_xor(last, o));
-  }
-  if (!live) return;
-  const double inv = amax > 0 ? 1.0 / amax : 0.0;
-  for (int u = lane; u < g.nu_pad; u += 64)
-    split_store(u < uend ? column(u) * inv : 0.0, Wh, Wl, W32, (long)r * g.nu_pad + u);
-  if (lane == 0) {
-    wscale[r] = amax;
+  for (int o = 32; o > 0; o >>= 1) last = max(last, __shfl_xor(last, o));
+  if (live && lane == 0) {
+    wscale[r] = ldexp(1.0, e);
     wsum[r] = tot;
     ulast[r] = last;
   }
