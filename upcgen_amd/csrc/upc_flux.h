@@ -58,12 +58,13 @@ __device__ __forceinline__ double node_b2(double b1, double b2, double c)
 // b >= 20); polarised: cos^2 and sin^2 weighted sums.
 //
 // Below the hadronic knee (b < ~11 fm for Pb-Pb) the G_AA spline holds only
-// the decaying ringing of the natural spline, |G| < 1e-30.  Wherever
-// |G| < gskip = p20 2^-70, P G is taken as 0 and the breakup spline is not
-// read: the caller subtracts p20 sum_k w_k from the same sum, and |P G| <=
-// |G| is below 2^-70 of that term, 10^5 times below the rounding that the
-// order of the pair sums already brings.  Below bskip (x < bskip2) the
-// bound holds for every b, so such nodes skip sqrt and G as well.
+// the decaying ringing of the natural spline, |G| < 1e-30.  Below bskip
+// (x < bskip2) |G| < gskip = p20 2^-70 holds for every b, and P G is taken
+// as 0 without sqrt, G or P: the caller subtracts p20 sum_k w_k from the
+// same sum, and |P G| <= |G| is below 2^-70 of that term, 10^5 times below
+// the rounding that the order of the pair sums already brings.  Above
+// bskip, modes 2-4 still skip the breakup spline wherever the G just
+// evaluated is below gskip (mode 1 has P = 1 and reads no breakup spline).
 template <bool POL>
 __device__ __forceinline__ void node_k(double b1, double b2, const double* cosk, const double* w5,
                                        int mode, double p20, double gskip, double bskip2, const double4* gaa,
