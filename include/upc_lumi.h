@@ -265,6 +265,12 @@ int upc_batch_autotune(upc_batch *batch, void *stream, upc_tune *h_out);
 /* Sets the launch shapes directly (values from upc_tune's candidate lists). */
 int upc_batch_set_tune(upc_batch *batch, int qags_blocks_per_cu, int lumi_waves);
 int upc_batch_info(upc_batch *batch, void *stream, upc_batch_stats *h_out);
+/* Per job of the last run: form-factor flux integrals that ended with a QAGS
+ * error or a form-factor spline domain error (h_failed[njobs]).  The
+ * reference process of that configuration stops at the first of them (GSL's
+ * default error handler, src/UpcCrossSection.cpp:209 and :188); the library
+ * keeps QUADPACK's value, NaN for a domain error.  Synchronises `stream`. */
+int upc_batch_job_failures(upc_batch *batch, void *stream, long long *h_failed);
 
 /* ---- learned surrogate of log L(M, Y; system) ----
  * No reference counterpart (the reference always integrates).  Two MLPs

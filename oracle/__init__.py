@@ -73,6 +73,13 @@ def lib():
             getattr(L, name).argtypes = [vp, d]
         L.orc_breakup_prob.restype = d
         L.orc_breakup_prob.argtypes = [vp, d, i]
+        L.orc_breakup_pxn.restype = d
+        L.orc_breakup_pxn.argtypes = [vp, d]
+        L.orc_breakup_tables.restype = i
+        L.orc_breakup_tables.argtypes = [vp, _DP, _DP, _DP, _DP, _DP, _DP]
+        L.orc_table_times.argtypes = [vp, _DP]
+        L.orc_flux_integrand.restype = d
+        L.orc_flux_integrand.argtypes = [vp, d, d, d]
         L.orc_flux_point.restype = d
         L.orc_flux_point.argtypes = [vp, d, d]
         L.orc_flux_form.restype = d
@@ -82,6 +89,8 @@ def lib():
         L.orc_lumi_pol.argtypes = [vp, d, d, _DP, _DP]
         L.orc_lumi_grid.restype = C.c_long
         L.orc_lumi_grid.argtypes = [vp, _DP, _DP, _DP]
+        L.orc_lumi_grid_counts.restype = C.c_long
+        L.orc_lumi_grid_counts.argtypes = [vp, _DP, _DP, _DP, C.POINTER(C.c_long)]
         for name in ("orc_dilep_cs_m_pol_s", "orc_dilep_cs_m_pol_ps"):
             getattr(L, name).restype = d
             getattr(L, name).argtypes = [i, d]
@@ -148,6 +157,23 @@ class Oracle:
     def formfac_spline(self, q2): return lib().orc_formfac_spline(self._ctx, q2)
     def breakup_prob(self, b, mode): return lib().orc_breakup_prob(self._ctx, b, mode)
     def breakup_spline(self, b): return lib().orc_breakup_spline(self._ctx, b)
+    def breakup_pxn(self, b): return lib().orc_breakup_pxn(self._ctx, b)
+
+    def table_times(self):
+        """Wall seconds of the context's table phases: G_AA, form factor, breakup."""
+        out = np.zeros(3)
+        lib().orc_table_times(self._ctx, _ptr(out))
+        return dict(gaa=out[0], form_factor=out[1], breakup=out[2])
+    def flux_integrand(self, b, k, q): return lib().orc_flux_integrand(self._ctx, b, k, q)
+
+    def breakup_tables(self):
+        """calcBreakupProb's tables: dict(ee, se) [1..n], eee, sa [1..161/160], zcon, o0."""
+        ee, se = np.zeros(10002), np.zeros(10002)
+        eee, sa = np.zeros(162), np.zeros(162)
+        zc, o0 = C.c_double(), C.c_double()
+        n = lib().orc_breakup_tables(self._ctx, _ptr(ee), _ptr(se), _ptr(eee), _ptr(sa), C.byref(zc),
+                                     C.byref(o0))
+        return dict(ee=ee[1:n + 1], se=se[1:n + 1], eee=eee[1:162], sa=sa[1:161], zcon=zc.value, o0=o0.value)
     def flux_point(self, b, k): return lib().orc_flux_point(self._ctx, b, k)
 
     def flux_form(self, b, k):
@@ -162,15 +188,23 @@ class Oracle:
         lib().orc_lumi_pol(self._ctx, M, Y, C.byref(s), C.byref(p))
         return s.value, p.value
 
-    def lumi_grid(self):
+    def lumi_grid(self, counts=False):
+        """The grid (or (Ls, Lp)) and the number of QAGS calls with ier != 0.
+        counts=True also returns an int64 array [nm][ny][4]: the integrand
+        evaluations of each bin's k1 and k2 flux rows and the number of
+        their integrals with ier != 0."""
         nm, ny = self.params.nm, self.params.ny
+        cnt = np.zeros((nm, ny, 4), dtype=np.int64)
+        cp = cnt.ctypes.data_as(C.POINTER(C.c_long))
+        dummy = np.zeros(1)
         if self.params.use_polarized:
-            s = np.zeros((nm, ny)); p = np.zeros((nm, ny)); dummy = np.zeros(1)
-            nbad = lib().orc_lumi_grid(self._ctx, _ptr(dummy), _ptr(s), _ptr(p))
-            return (s, p), nbad
-        out = np.zeros((nm, ny)); dummy = np.zeros(1)
-        nbad = lib().orc_lumi_grid(self._ctx, _ptr(out), _ptr(dummy), _ptr(dummy))
-        return out, nbad
+            s = np.zeros((nm, ny)); p = np.zeros((nm, ny))
+            nbad = lib().orc_lumi_grid_counts(self._ctx, _ptr(dummy), _ptr(s), _ptr(p), cp)
+            res = (s, p)
+        else:
+            res = np.zeros((nm, ny))
+            nbad = lib().orc_lumi_grid_counts(self._ctx, _ptr(res), _ptr(dummy), _ptr(dummy), cp)
+        return (res, nbad, cnt) if counts else (res, nbad)
 
     def cs_zm(self, flag=0):
         out = np.zeros((self.params.nm, self.params.nz))

@@ -34,6 +34,7 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -682,7 +683,17 @@ struct orc_ctx {
   double ee[10002], eee[163], se[10002], sa[162];
   double zcon, o0;
   int nee; /* index of the 99999999999 sentinel */
+  /* wall seconds of the table phases of orc_create: rho0 + G_AA, form
+   * factor, breakup (first-call tables + the 10^6-knot spline) */
+  double t_gaa, t_ff, t_breakup;
 };
+
+static double wall_s(void)
+{
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
 
 /* src/UpcCrossSection.cpp:152-163 */
 static double calc_ws_rho(double R, double a, int A)
@@ -876,16 +887,16 @@ static void breakup_init(orc_ctx *c)
   c->nee = ij + 1;
 }
 
-/* src/UpcCrossSection.cpp:972-1018 (the unused one-neutron sum p1n is skipped:
- * it has no effect on the returned value). */
-double orc_breakup_prob(const orc_ctx *c, double b, int mode)
+/* src/UpcCrossSection.cpp:972-992: the XN trapezoid sum p_xn(b); -1 where
+ * the reference returns before summing (omax < o0). */
+double orc_breakup_pxn(const orc_ctx *c, double b)
 {
   const double hbarcmev = 197.3269718;
   const double gammatarg = 2. * c->g1 * c->g1 - 1.;
   const double omaxx = c->g1 > 500. ? 1.E10 : 1.E7;
   double pxn = 0.;
   double omax = fmin(omaxx, 4. * gammatarg * (hbarcmev) / b);
-  if (omax < c->o0) return 0.;
+  if (omax < c->o0) return -1.;
   double gk1m = orc_bessel_k1_as(c->ee[1] * b / ((hbarcmev)*gammatarg));
   int k = 2;
   while (c->ee[k] < omax) {
@@ -895,12 +906,37 @@ double orc_breakup_prob(const orc_ctx *c, double b, int mode)
     k = k + 1;
     gk1m = gk1;
   }
+  return pxn;
+}
+
+/* src/UpcCrossSection.cpp:972-1018 (the unused one-neutron sum p1n is skipped:
+ * it has no effect on the returned value). */
+double orc_breakup_prob(const orc_ctx *c, double b, int mode)
+{
+  const double pxn = orc_breakup_pxn(c, b);
+  if (pxn < 0.) return 0.;
   double P = 0.;
   if (mode == 1) P = 1.;
   if (mode == 2) P = (1 - exp(-1 * pxn)) * (1 - exp(-1 * pxn));
   if (mode == 3) P = exp(-2 * pxn);
   if (mode == 4) P = 2. * exp(-pxn) * (1. - exp(-pxn));
   return P;
+}
+
+int orc_breakup_tables(const orc_ctx *c, double *ee, double *se, double *eee, double *sa, double *zcon,
+                       double *o0)
+{
+  for (int i = 0; i <= c->nee; ++i) {
+    ee[i] = c->ee[i];
+    se[i] = c->se[i];
+  }
+  for (int i = 0; i <= 161; ++i) {
+    eee[i] = c->eee[i];
+    sa[i] = c->sa[i];
+  }
+  *zcon = c->zcon;
+  *o0 = c->o0;
+  return c->nee;
 }
 
 /* src/UpcCrossSection.cpp:416-434 (heap arrays instead of 8 MB stack arrays) */
@@ -935,12 +971,25 @@ orc_ctx *orc_create(const orc_params *p)
   /* src/UpcCrossSection.cpp:120-124 */
   c->factor = p->Z * p->Z * kAlpha / M_PI / M_PI / kHc / kHc;
   c->mNucl = (p->Z * kMProt + (p->A - p->Z) * kMNeut) / p->A;
+  double t = wall_s();
   c->rho0 = calc_ws_rho(p->R, p->a, p->A);
   prepare_gaa(c);
+  c->t_gaa = wall_s() - t;
+  t = wall_s();
   prepare_form_fac(c);
+  c->t_ff = wall_s() - t;
+  t = wall_s();
   breakup_init(c);
   if (p->breakup_mode > 1) prepare_breakup(c);
+  c->t_breakup = wall_s() - t;
   return c;
+}
+
+void orc_table_times(const orc_ctx *c, double *out)
+{
+  out[0] = c->t_gaa;
+  out[1] = c->t_ff;
+  out[2] = c->t_breakup;
 }
 
 void orc_destroy(orc_ctx *c)
@@ -998,6 +1047,12 @@ static double flux_form_integrand(double q, void *vp)
   return q * q * ff / t * j1(fp->b * q / kHc);
 }
 
+double orc_flux_integrand(const orc_ctx *c, double b, double k, double q)
+{
+  ff_par fp = {c, b, k, c->g1, 0};
+  return flux_form_integrand(q, &fp);
+}
+
 /* src/UpcCrossSection.cpp:194-218 */
 double orc_flux_form(const orc_ctx *c, double b, double k, int *neval, int *ier)
 {
@@ -1016,18 +1071,23 @@ double orc_flux_form(const orc_ctx *c, double b, double k, int *neval, int *ier)
   return c->factor * Q * Q / k;
 }
 
-static double flux_form_checked(const orc_ctx *c, double b, double k, long *nbad)
+/* QAGS bookkeeping of one luminosity bin, per flux row (k1, k2): integrand
+ * evaluations and integrals that ended with ier != 0. */
+typedef struct { long ev1, ev2, bad1, bad2; } orc_cnt;
+
+static double flux_form_checked(const orc_ctx *c, double b, double k, long *nbad, long *nev)
 {
-  int ier = 0;
-  const double f = orc_flux_form(c, b, k, NULL, &ier);
+  int ier = 0, ne = 0;
+  const double f = orc_flux_form(c, b, k, &ne, &ier);
   if (ier) (*nbad)++;
+  if (nev) *nev += ne;
   return f;
 }
 
 static double bnode(double bmin, double logd, int i) { return bmin * exp(i * logd); }
 
 /* src/UpcCrossSection.cpp:221-271 */
-static double lumi_impl(const orc_ctx *c, double M, double Y, long *nbad)
+static double lumi_impl(const orc_ctx *c, double M, double Y, orc_cnt *cnt)
 {
   const double R = c->p.R;
   const int isPoint = c->p.flux_point;
@@ -1044,7 +1104,7 @@ static double lumi_impl(const orc_ctx *c, double M, double Y, long *nbad)
     const double b2l = b2min * exp(j * log_delta_b2);
     const double b2h = b2min * exp((j + 1.) * log_delta_b2);
     const double b2 = (b2h + b2l) / 2.;
-    flux[j] = flux_form_checked(c, b2, k2, nbad);
+    flux[j] = flux_form_checked(c, b2, k2, &cnt->bad2, &cnt->ev2);
   }
   double cphi[10];
   for (int k = 0; k < 10; ++k) cphi[k] = cos(M_PI * kGLx[k]);
@@ -1070,13 +1130,13 @@ static double lumi_impl(const orc_ctx *c, double M, double Y, long *nbad)
       }
       sum_b2 += flux[j] * sum_phi * b2 * (b2h - b2l);
     }
-    sum += flux_form_checked(c, b1, k1, nbad) * sum_b2 * b1 * (b1h - b1l);
+    sum += flux_form_checked(c, b1, k1, &cnt->bad1, &cnt->ev1) * sum_b2 * b1 * (b1h - b1l);
   }
   return 2 * M_PI * M_PI * M * sum;
 }
 
 /* src/UpcCrossSection.cpp:274-335 */
-static void lumi_pol_impl(const orc_ctx *c, double M, double Y, double *ns, double *np, long *nbad)
+static void lumi_pol_impl(const orc_ctx *c, double M, double Y, double *ns, double *np, orc_cnt *cnt)
 {
   const double R = c->p.R;
   const int isPoint = c->p.flux_point;
@@ -1093,7 +1153,7 @@ static void lumi_pol_impl(const orc_ctx *c, double M, double Y, double *ns, doub
     const double b2l = b2min * exp(j * log_delta_b2);
     const double b2h = b2min * exp((j + 1) * log_delta_b2);
     const double b2 = (b2h + b2l) / 2.;
-    flux[j] = flux_form_checked(c, b2, k2, nbad);
+    flux[j] = flux_form_checked(c, b2, k2, &cnt->bad2, &cnt->ev2);
   }
   double cph[10], sph[10];
   for (int k = 0; k < 10; ++k) {
@@ -1106,7 +1166,7 @@ static void lumi_pol_impl(const orc_ctx *c, double M, double Y, double *ns, doub
     const double b1l = b1min * exp(i * log_delta_b1);
     const double b1h = b1min * exp((i + 1) * log_delta_b1);
     const double b1 = (b1h + b1l) / 2.;
-    const double ff_b1 = flux_form_checked(c, b1, k1, nbad);
+    const double ff_b1 = flux_form_checked(c, b1, k1, &cnt->bad1, &cnt->ev1);
     double sum_b2_s = 0, sum_b2_p = 0;
     for (int j = 0; j < NB2; ++j) {
       const double b2l = b2min * exp(j * log_delta_b2);
@@ -1136,14 +1196,14 @@ static void lumi_pol_impl(const orc_ctx *c, double M, double Y, double *ns, doub
 
 double orc_lumi(const orc_ctx *c, double M, double Y)
 {
-  long nbad = 0;
-  return lumi_impl(c, M, Y, &nbad);
+  orc_cnt cnt = {0, 0, 0, 0};
+  return lumi_impl(c, M, Y, &cnt);
 }
 
 void orc_lumi_pol(const orc_ctx *c, double M, double Y, double *ns, double *np)
 {
-  long nbad = 0;
-  lumi_pol_impl(c, M, Y, ns, np, &nbad);
+  orc_cnt cnt = {0, 0, 0, 0};
+  lumi_pol_impl(c, M, Y, ns, np, &cnt);
 }
 
 /* calcPhotonFlux, src/UpcCrossSection.cpp:699-720: photon flux of one
@@ -1165,7 +1225,7 @@ double orc_photon_flux(const orc_ctx *c, double M, double Y)
     double breakup = 1.;
     if (c->p.breakup_mode != 1) breakup = orc_spline_eval(&c->breakp, b < 20. ? b : 20., NULL);
     const double gaa = b < 20. ? orc_spline_eval(&c->gaa, b, NULL) : 1.;
-    sum += breakup * gaa * flux_form_checked(c, b, k, &nbad) * b * (bh - bl);
+    sum += breakup * gaa * flux_form_checked(c, b, k, &nbad, NULL) * b * (bh - bl);
   }
   return 2. * M_PI * k * sum;
 }
@@ -1193,8 +1253,10 @@ void orc_nuc_cs_y(const orc_ctx *c, double mpart, const double *cs_pos, const do
 }
 
 /* src/UpcCrossSection.cpp:494-576: static contiguous M-row block per thread,
- * lower-edge evaluation, value x dm x dy. */
-long orc_lumi_grid(const orc_ctx *c, double *out, double *outS, double *outP)
+ * lower-edge evaluation, value x dm x dy.  counts (may be NULL), [im][iy][4]:
+ * integrand evaluations of the bin's k1 and k2 flux rows, then the number of
+ * their integrals that ended with ier != 0. */
+long orc_lumi_grid_counts(const orc_ctx *c, double *out, double *outS, double *outP, long *counts)
 {
   const int nm = c->p.nm, ny = c->p.ny;
   const double dy = (c->p.ymax - c->p.ymin) / ny;
@@ -1215,19 +1277,33 @@ long orc_lumi_grid(const orc_ctx *c, double *out, double *outS, double *outP)
       const double m = c->p.mmin + dm * im;
       for (int iy = 0; iy < ny; ++iy) {
         const double y = c->p.ymin + dy * iy;
+        orc_cnt cnt = {0, 0, 0, 0};
         if (c->p.use_polarized) {
           double ls, lp;
-          lumi_pol_impl(c, m, y, &ls, &lp, &nbad);
+          lumi_pol_impl(c, m, y, &ls, &lp, &cnt);
           outS[(long)im * ny + iy] = ls * dm * dy;
           outP[(long)im * ny + iy] = lp * dm * dy;
         } else {
-          out[(long)im * ny + iy] = lumi_impl(c, m, y, &nbad) * dm * dy;
+          out[(long)im * ny + iy] = lumi_impl(c, m, y, &cnt) * dm * dy;
+        }
+        nbad += cnt.bad1 + cnt.bad2;
+        if (counts) {
+          long *q = counts + 4 * ((long)im * ny + iy);
+          q[0] = cnt.ev1;
+          q[1] = cnt.ev2;
+          q[2] = cnt.bad1;
+          q[3] = cnt.bad2;
         }
       }
     }
     nbad_total += nbad;
   }
   return nbad_total;
+}
+
+long orc_lumi_grid(const orc_ctx *c, double *out, double *outS, double *outP)
+{
+  return orc_lumi_grid_counts(c, out, outS, outP, NULL);
 }
 
 /* ------------------------------------------------------------------------ */

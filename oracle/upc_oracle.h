@@ -60,6 +60,9 @@ void orc_spline_free(orc_spline *s);
 /* ---- context: UpcCrossSection::init (src/UpcCrossSection.cpp:116-137) ---- */
 orc_ctx *orc_create(const orc_params *p);   /* rho0, G_AA, FF and breakup tables */
 void orc_destroy(orc_ctx *c);
+/* Wall seconds orc_create spent on the tables: out[0] rho0 + G_AA, out[1]
+ * the form-factor spline, out[2] the breakup tables and spline. */
+void orc_table_times(const orc_ctx *c, double *out);
 
 void orc_set_qags_tol(orc_ctx *c, double epsabs, double epsrel);
 double orc_rho0(const orc_ctx *c);
@@ -72,6 +75,16 @@ double orc_formfac(const orc_ctx *c, double Q2);        /* analytic F(Q2) */
 double orc_formfac_spline(const orc_ctx *c, double Q2); /* tabulated F */
 double orc_breakup_prob(const orc_ctx *c, double b, int mode); /* Starlight sum */
 double orc_breakup_spline(const orc_ctx *c, double b);  /* tabulated P(b) */
+/* The XN sum p_xn(b) of calcBreakupProb (src/UpcCrossSection.cpp:972-992);
+ * -1 where the reference returns 0 before summing. */
+double orc_breakup_pxn(const orc_ctx *c, double b);
+/* calcBreakupProb's first-call tables (src/UpcCrossSection.cpp:852-969):
+ * ee[1..n], se[1..n] (n returned; ee[n] is the 99999999999 sentinel; arrays
+ * of >= 10002), eee[1..161], sa[1..160] (>= 162), zcon and o0. */
+int orc_breakup_tables(const orc_ctx *c, double *ee, double *se, double *eee, double *sa, double *zcon,
+                       double *o0);
+/* fluxFormIntegrand(q) at (b, k) (src/UpcCrossSection.cpp:181-191). */
+double orc_flux_integrand(const orc_ctx *c, double b, double k, double q);
 double orc_flux_point(const orc_ctx *c, double b, double k);
 double orc_flux_form(const orc_ctx *c, double b, double k, int *neval, int *ier);
 double orc_lumi(const orc_ctx *c, double M, double Y);
@@ -80,6 +93,11 @@ void orc_lumi_pol(const orc_ctx *c, double M, double Y, double *ns, double *np);
  * out (unpolarised) or outS/outP (polarised).  Returns the number of QAGS
  * calls that ended with ier != 0 (the reference would abort on any). */
 long orc_lumi_grid(const orc_ctx *c, double *out, double *outS, double *outP);
+/* The same grid; counts[(im*ny+iy)*4 + 0..3] (may be NULL) receive the QAGS
+ * integrand evaluations of the bin's k1 and k2 flux rows (120 fluxForm calls
+ * each, src/UpcCrossSection.cpp:235-269) and how many of those integrals
+ * ended with ier != 0 (k1, k2). */
+long orc_lumi_grid_counts(const orc_ctx *c, double *out, double *outS, double *outP, long *counts);
 /* calcPhotonFlux(M, Y) and calcNucCrossSectionY (src/UpcCrossSection.cpp:699-748). */
 double orc_photon_flux(const orc_ctx *c, double M, double Y);
 void orc_nuc_cs_y(const orc_ctx *c, double mpart, const double *cs_pos, const double *cs_neg, double *cs,

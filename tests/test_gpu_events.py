@@ -29,15 +29,20 @@ def test_ditau_events_1e8(gpu_ctx):
     import torch
     from scipy.stats import chi2
     g = gpu_ctx(**CFG)
+    # Four (Y, M) bins of this grid are NaN: QK21 nodes of their form-factor
+    # integrals fall past the form-factor table's last knot, where the
+    # reference's spline raises GSL_EDOM and its process stops.  A caller that
+    # samples on regardless has to empty them; so does this test.
     # tables the draws use: MFMA luminosity
     Lm = g.lumi_grid(U.FORM_MFMA)
-    cs_m, _, _ = g.nuc_cs_ym(Lm)
+    assert int(torch.isnan(Lm).sum()) == 4
+    cs_m, _, _ = g.nuc_cs_ym(torch.nan_to_num(Lm, nan=0.0))
     sum_ym = g.pdf_init(cs_m.reshape(1, -1))[0].contiguous()
     zt = g.cs_zm(0)
     sum_z = g.pdf_init(zt).contiguous()
     # expected probabilities: FP64 luminosity
     L64 = g.lumi_grid(U.FORM_FP64)
-    cs64, _, _ = g.nuc_cs_ym(L64)
+    cs64, _, _ = g.nuc_cs_ym(torch.nan_to_num(L64, nan=0.0))
     c = cs64.cpu().numpy()  # [ny][nm]
     py = c.sum(axis=1) / c.sum()
     pm = c.sum(axis=0) / c.sum()

@@ -199,3 +199,35 @@ def test_cli_mfma_form(tmp_path):
     assert "different parameters" in out and "Starting all over" in out
     tot = logged(out, "totCS_mb")
     assert tot == pytest.approx(tot64, rel=1e-5) and tot != tot64
+
+
+def test_facade_members_beyond_the_cli(oracle_ctx):
+    """calcPhotonFlux, calcNucCrossSectionY (with a test elementary sigma(y)),
+    getPhotonPt and getPairMomentum of the C++ facade (driven by
+    tests/facade_members.cpp) against the oracle: the vector-meson photon flux
+    and sigma(y) to 1e-9, the photon-pt draws' mean against the reference
+    histogram's (5 standard errors), and the pair kinematics exactly."""
+    import json
+    exe = os.path.join(ROOT, "tests", "facade_members")
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    d = json.loads([l for l in r.stdout.splitlines() if l.startswith("{")][-1])
+    o = oracle_ctx(flux_point=0, breakup_mode=2, nm=3, ny=9, ymin=-4.0, ymax=4.0)
+    for y, v in d["photon_flux"]:
+        ref = o.photon_flux(3.0969, y)
+        assert v == pytest.approx(ref, rel=1e-9), y
+    ys = -4.0 + 8.0 / 9 * np.arange(9)
+    sig = lambda y: 1.0 + 0.1 * y + 0.01 * y * y
+    cs, ratio, tot = o.nuc_cs_y(3.0969, sig(ys), sig(-ys))
+    assert np.allclose(d["cs_y"], cs, rtol=1e-9, atol=0) and np.allclose(d["ratio_y"], ratio, rtol=1e-9, atol=0)
+    assert d["tot_y"] == pytest.approx(tot, rel=1e-9)
+    w = o.photon_pt_hist(2.0, d["gtot"])
+    ptmax = 6 * 0.1973269718 / 6.68
+    centers = (np.arange(5000) + 0.5) * ptmax / 5000
+    mean = np.sum(w * centers) / w.sum()
+    rms = np.sqrt(np.sum(w * centers ** 2) / w.sum())
+    print(f"photon pt at 2 GeV: draws mean {d['pt_mean']:.5f} rms {d['pt_rms']:.5f}; "
+          f"histogram {mean:.5f} {rms:.5f}")
+    assert abs(d["pt_mean"] - mean) < 5 * rms / np.sqrt(4000)
+    assert d["pair_max_dm"] < 1e-9 and d["pair_max_dy"] < 1e-9
+    assert d["pair_mean_pt"] > 0 and d["pair_pt_zero"] == 0.0

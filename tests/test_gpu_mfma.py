@@ -81,27 +81,6 @@ def test_mfma_modes_vs_oracle(gpu_ctx, oracle_ctx, kw):
             _check(f"{NAMES[form]} {kw}", _np(out), ref, tol)
 
 
-def test_mfma_config2_full_grid(gpu_ctx, oracle_ctx):
-    """1001 x 121 XNXN form-factor grid: every bin against the FP64 GPU form
-    (itself within 1e-9 of the oracle), and a sample of bins against the
-    oracle directly."""
-    g = gpu_ctx(flux_point=0, breakup_mode=2, nm=1001, ny=121, nz=100)
-    L64 = _np(g.lumi_grid(U.FORM_FP64))
-    o = oracle_ctx(flux_point=0, breakup_mode=2, nm=1001, ny=121, nz=100)
-    rng = np.random.default_rng(9)
-    dm, dy = (50 - 3.56) / 1001, 12.0 / 121
-    ims = np.concatenate([rng.integers(0, 1001, 12), [0, 1000, 1000, 500]])
-    iys = np.concatenate([rng.integers(0, 121, 12), [120, 120, 0, 60]])
-    ref = np.array([o.lumi(3.56 + dm * i, -6.0 + dy * j) * dm * dy for i, j in zip(ims, iys)])
-    for form in FORMS:
-        L = _np(g.lumi_grid(form))
-        assert np.all(np.isfinite(L)) and np.all(L > 0)
-        e = _check(f"{NAMES[form]} 1001x121 vs FP64 form", L, L64)
-        im, iy = np.unravel_index(np.argmax(e), e.shape)
-        print(f"  worst bin im={im} iy={iy}: {e[im, iy]:.2e}; ksteps {g.info.mfma_ksteps}")
-        _check(f"{NAMES[form]} 1001x121 sampled vs oracle", L[ims, iys], ref)
-
-
 def test_mfma_spacing_controls_error(gpu_ctx, oracle_ctx):
     kw = dict(flux_point=0, breakup_mode=2, nm=9, ny=7)
     g = gpu_ctx(**kw)

@@ -213,26 +213,6 @@ def test_lumi_points_arbitrary(gpu_ctx, oracle_ctx):
     _lumi_check(L, [o.lumi(m, y) for m, y in zip(M, Y)])
 
 
-def test_config2_full_grid_properties(gpu_ctx, oracle_ctx):
-    """1001 x 121 XNXN form-factor grid: size-independent properties plus a
-    random sample of bins against the oracle."""
-    g = gpu_ctx(flux_point=0, breakup_mode=2, nm=1001, ny=121, nz=100)
-    L = _np(g.lumi_grid())
-    inf = g.info
-    print(f"qags calls {inf.qags_calls} evals {inf.qags_evals} bad {inf.qags_bad}")
-    assert inf.qags_bad == 0
-    assert np.all(np.isfinite(L)) and np.all(L > 0)
-    # y_iy and y_{ny-iy} = -y_iy (lower bin edges): L symmetric
-    ny = 121
-    assert rel(L[:, 1:], L[:, ny - np.arange(1, ny)]) < 1e-9
-    o = oracle_ctx(flux_point=0, breakup_mode=2, nm=1001, ny=121, nz=100)
-    rng = np.random.default_rng(5)
-    dm, dy = (50 - 3.56) / 1001, 12.0 / 121
-    ims, iys = rng.integers(0, 1001, 24), rng.integers(0, 121, 24)
-    ref = [o.lumi(3.56 + dm * i, -6.0 + dy * j) * dm * dy for i, j in zip(ims, iys)]
-    _lumi_check(L[ims, iys], ref)
-
-
 def test_cross_section_tables(gpu_ctx, oracle_ctx):
     import oracle as O
     for pol in (0, 1):

@@ -125,6 +125,7 @@ SIGNATURES = {
     "upc_batch_autotune": (_i, [_vp, _vp, C.POINTER(Tune)]),
     "upc_batch_set_tune": (_i, [_vp, _i, _i]),
     "upc_batch_info": (_i, [_vp, _vp, C.POINTER(BatchStats)]),
+    "upc_batch_job_failures": (_i, [_vp, _vp, C.POINTER(_ll)]),
     "upc_sur_create": (_i, [_ull, C.POINTER(_vp)]),
     "upc_sur_destroy": (_i, [_vp]),
     "upc_sur_param_count": (_i, [C.POINTER(_i)]),
@@ -164,13 +165,26 @@ def _check(rc: int, what: str):
         raise UpcError(f"{what} failed ({rc}): {load().upc_last_error().decode()}")
 
 
-def _dptr(t):
+def _dptr(t, dtype=None, numel=None, device=None, name="tensor"):
+    """Device pointer of a tensor after checking what the C call will assume:
+    a contiguous device tensor, of `dtype` (float64 unless stated), holding
+    at least `numel` elements, on `device` when given.  A mismatch raises
+    UpcError before any kernel runs (the C ABI takes plain pointers and would
+    read or write past the end of a shorter or narrower buffer)."""
     if t is None:
         return None
-    if not t.is_cuda:
-        raise UpcError("device tensor expected")
+    import torch
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise UpcError(f"{name}: device tensor expected")
     if not t.is_contiguous():
-        raise UpcError("contiguous tensor expected")
+        raise UpcError(f"{name}: contiguous tensor expected")
+    want = torch.float64 if dtype is None else dtype
+    if t.dtype != want:
+        raise UpcError(f"{name}: dtype {t.dtype}, expected {want}")
+    if numel is not None and t.numel() < numel:
+        raise UpcError(f"{name}: {t.numel()} elements, at least {numel} expected")
+    if device is not None and t.device.index != device:
+        raise UpcError(f"{name}: on cuda:{t.device.index}, the context lives on cuda:{device}")
     return C.c_void_p(t.data_ptr())
 
 
@@ -189,6 +203,7 @@ class Context:
         if not torch.cuda.is_available():
             raise UpcError("no HIP device visible: upcgen_amd runs on the GPU only")
         self.config = config if config is not None else Config.make(**kw)
+        self.device = torch.cuda.current_device()
         self._ctx = C.c_void_p()
         _check(load().upc_ctx_create(C.byref(self.config), _stream(stream), C.byref(self._ctx)),
                "upc_ctx_create")
@@ -240,59 +255,69 @@ class Context:
     # --- batched per-point functions (torch float64 cuda tensors) ---
     def _empty(self, n, dtype=None):
         import torch
-        return torch.empty(n, dtype=dtype or torch.float64, device="cuda")
+        return torch.empty(n, dtype=dtype or torch.float64, device=f"cuda:{self.device}")
+
+    def _in(self, t, numel=None, name="input", dtype=None):
+        return _dptr(t, dtype=dtype, numel=numel, device=self.device, name=name)
+
+    def _pair(self, a, b, names):
+        """Two per-point inputs of equal length: (n, ptr_a, ptr_b)."""
+        n = a.numel()
+        if b is None or b.numel() != n:
+            raise UpcError(f"{names[0]} and {names[1]} must have the same number of elements")
+        return n, self._in(a, n, names[0]), self._in(b, n, names[1])
 
     def flux_point(self, b, k, stream=None):
-        out = self._empty(b.numel())
-        _check(load().upc_flux_point(self._ctx, _dptr(b), _dptr(k), b.numel(), _dptr(out),
-                                     _stream(stream)), "upc_flux_point")
+        n, pb, pk = self._pair(b, k, ("b", "k"))
+        out = self._empty(n)
+        _check(load().upc_flux_point(self._ctx, pb, pk, n, _dptr(out), _stream(stream)), "upc_flux_point")
         return out
 
     def flux_form(self, b, k, stream=None):
         import torch
-        out = self._empty(b.numel())
-        ier = self._empty(b.numel(), torch.int32)
-        _check(load().upc_flux_form(self._ctx, _dptr(b), _dptr(k), b.numel(), _dptr(out), _dptr(ier),
+        n, pb, pk = self._pair(b, k, ("b", "k"))
+        out = self._empty(n)
+        ier = self._empty(n, torch.int32)
+        _check(load().upc_flux_form(self._ctx, pb, pk, n, _dptr(out), _dptr(ier, torch.int32),
                                     _stream(stream)), "upc_flux_form")
         return out, ier
 
     def form_factor(self, q2, tabulated=False, stream=None):
         out = self._empty(q2.numel())
-        _check(load().upc_form_factor(self._ctx, _dptr(q2), q2.numel(), int(tabulated), _dptr(out),
+        _check(load().upc_form_factor(self._ctx, self._in(q2, name="q2"), q2.numel(), int(tabulated), _dptr(out),
                                       _stream(stream)), "upc_form_factor")
         return out
 
     def breakup_prob(self, b, mode=None, tabulated=False, stream=None):
         out = self._empty(b.numel())
         m = self.config.breakup_mode if mode is None else mode
-        _check(load().upc_breakup_prob(self._ctx, _dptr(b), b.numel(), m, int(tabulated), _dptr(out),
+        _check(load().upc_breakup_prob(self._ctx, self._in(b, name="b"), b.numel(), m, int(tabulated), _dptr(out),
                                        _stream(stream)), "upc_breakup_prob")
         return out
 
     def nuclear_overlap(self, b, which=0, stream=None):
         out = self._empty(b.numel())
-        _check(load().upc_nuclear_overlap(self._ctx, _dptr(b), b.numel(), which, _dptr(out),
+        _check(load().upc_nuclear_overlap(self._ctx, self._in(b, name="b"), b.numel(), which, _dptr(out),
                                           _stream(stream)), "upc_nuclear_overlap")
         return out
 
     def lumi_points(self, M, Y, stream=None):
-        n = M.numel()
+        n, pm, py = self._pair(M, Y, ("M", "Y"))
         if self.config.use_polarized:
             ls, lp = self._empty(n), self._empty(n)
-            _check(load().upc_lumi_points(self._ctx, _dptr(M), _dptr(Y), n, None, _dptr(ls), _dptr(lp),
+            _check(load().upc_lumi_points(self._ctx, pm, py, n, None, _dptr(ls), _dptr(lp),
                                           _stream(stream)), "upc_lumi_points")
             return ls, lp
         out = self._empty(n)
-        _check(load().upc_lumi_points(self._ctx, _dptr(M), _dptr(Y), n, _dptr(out), None, None,
+        _check(load().upc_lumi_points(self._ctx, pm, py, n, _dptr(out), None, None,
                                       _stream(stream)), "upc_lumi_points")
         return out
 
     def photon_flux(self, M, Y, stream=None):
         """calcPhotonFlux(M, Y) at n points (vector-meson photon flux)."""
-        n = M.numel()
+        n, pm, py = self._pair(M, Y, ("M", "Y"))
         out = self._empty(n)
-        _check(load().upc_photon_flux(self._ctx, _dptr(M), _dptr(Y), n, _dptr(out), _stream(stream)),
-               "upc_photon_flux")
+        _check(load().upc_photon_flux(self._ctx, pm, py, n, _dptr(out), _stream(stream)), "upc_photon_flux")
         return out
 
     def nuc_cs_y(self, mpart, cs_pos, cs_neg, stream=None):
@@ -300,7 +325,8 @@ class Context:
         ny = self.config.ny
         cs, ratio = self._empty(ny), self._empty(ny)
         tot = C.c_double(0.0)
-        _check(load().upc_nuc_cs_y(self._ctx, mpart, _dptr(cs_pos), _dptr(cs_neg), _dptr(cs), _dptr(ratio),
+        _check(load().upc_nuc_cs_y(self._ctx, mpart, self._in(cs_pos, ny, "cs_pos"), self._in(cs_neg, ny, "cs_neg"),
+                                   _dptr(cs), _dptr(ratio),
                                    C.byref(tot), _stream(stream)), "upc_nuc_cs_y")
         return cs, ratio, tot.value
 
@@ -309,15 +335,16 @@ class Context:
         """prepareTwoPhotonLumi grid [nm][ny] (x dm x dy). Polarised -> (Ls, Lp)."""
         import torch
         nm, ny = self.config.nm, self.config.ny
+        dev = f"cuda:{self.device}"
         if self.config.use_polarized:
             ls, lp = out if out is not None else (
-                torch.empty((nm, ny), dtype=torch.float64, device="cuda"),
-                torch.empty((nm, ny), dtype=torch.float64, device="cuda"))
-            _check(load().upc_lumi_grid(self._ctx, form, None, _dptr(ls), _dptr(lp), _stream(stream)),
-                   "upc_lumi_grid")
+                torch.empty((nm, ny), dtype=torch.float64, device=dev),
+                torch.empty((nm, ny), dtype=torch.float64, device=dev))
+            _check(load().upc_lumi_grid(self._ctx, form, None, self._in(ls, nm * ny, "out Ls"),
+                                        self._in(lp, nm * ny, "out Lp"), _stream(stream)), "upc_lumi_grid")
             return ls, lp
-        L = out if out is not None else torch.empty((nm, ny), dtype=torch.float64, device="cuda")
-        _check(load().upc_lumi_grid(self._ctx, form, _dptr(L), None, None, _stream(stream)),
+        L = out if out is not None else torch.empty((nm, ny), dtype=torch.float64, device=dev)
+        _check(load().upc_lumi_grid(self._ctx, form, self._in(L, nm * ny, "out"), None, None, _stream(stream)),
                "upc_lumi_grid")
         return L
 
@@ -344,7 +371,10 @@ class Context:
     def elem_cs(self, flag, m, z=None, stream=None):
         """Elementary sigma: flag 0/1/2 = sigma(M), S, PS; 3/4/5 = dsigma/dz(z, M), S, PS."""
         out = self._empty(m.numel())
-        _check(load().upc_elem_cs(self._ctx, flag, _dptr(m), _dptr(z), m.numel(), _dptr(out),
+        if flag >= 3 and (z is None or z.numel() != m.numel()):
+            raise UpcError("elem_cs: z must have as many elements as m")
+        _check(load().upc_elem_cs(self._ctx, flag, self._in(m, name="m"), self._in(z, m.numel(), "z"), m.numel(),
+                                  _dptr(out),
                                   _stream(stream)), "upc_elem_cs")
         return out
 
@@ -352,16 +382,17 @@ class Context:
         """calcNucCrossSectionYM: returns (cs [ny][nm], ratio or None, totCS [mb])."""
         import torch
         nm, ny = self.config.nm, self.config.ny
-        cs = torch.empty((ny, nm), dtype=torch.float64, device="cuda")
+        dev = f"cuda:{self.device}"
+        cs = torch.empty((ny, nm), dtype=torch.float64, device=dev)
         tot = C.c_double(tot_cs)
         if self.config.use_polarized:
             ls, lp = lumi
-            ratio = torch.empty((ny, nm), dtype=torch.float64, device="cuda")
-            _check(load().upc_nuc_cs_ym(self._ctx, None, _dptr(ls), _dptr(lp), _dptr(cs), _dptr(ratio),
-                                        C.byref(tot), _stream(stream)), "upc_nuc_cs_ym")
+            ratio = torch.empty((ny, nm), dtype=torch.float64, device=dev)
+            _check(load().upc_nuc_cs_ym(self._ctx, None, self._in(ls, nm * ny, "Ls"), self._in(lp, nm * ny, "Lp"),
+                                        _dptr(cs), _dptr(ratio), C.byref(tot), _stream(stream)), "upc_nuc_cs_ym")
             return cs, ratio, tot.value
-        _check(load().upc_nuc_cs_ym(self._ctx, _dptr(lumi), None, None, _dptr(cs), None, C.byref(tot),
-                                    _stream(stream)), "upc_nuc_cs_ym")
+        _check(load().upc_nuc_cs_ym(self._ctx, self._in(lumi, nm * ny, "lumi"), None, None, _dptr(cs), None,
+                                    C.byref(tot), _stream(stream)), "upc_nuc_cs_ym")
         return cs, None, tot.value
 
     @staticmethod
@@ -369,18 +400,31 @@ class Context:
         """gsl_histogram_pdf_init per row of a 2-D tensor (rows, n) -> (rows, n+1)."""
         import torch
         b2 = bins.reshape(-1, bins.shape[-1]) if bins.dim() > 1 else bins.reshape(1, -1)
+        b2 = b2.contiguous()
         rows, n = b2.shape
-        out = torch.empty((rows, n + 1), dtype=torch.float64, device="cuda")
-        _check(load().upc_pdf_init(_dptr(b2.contiguous()), rows, n, _dptr(out), _stream(stream)),
+        out = torch.empty((rows, n + 1), dtype=torch.float64, device=b2.device)
+        _check(load().upc_pdf_init(_dptr(b2, name="bins"), rows, n, _dptr(out), _stream(stream)),
                "upc_pdf_init")
         return out
+
+    def _tables(self, sum_ym, sum_z, sum_zs, sum_zp, ratio):
+        """Pointers to the sampler tables, checked against the grid sizes."""
+        nm, ny, nz = self.config.nm, self.config.ny, self.config.nz
+        if self.config.use_polarized:
+            if sum_zs is None or sum_zp is None or ratio is None:
+                raise UpcError("polarised sampling needs sum_zs, sum_zp and ratio")
+        elif sum_z is None:
+            raise UpcError("sampling needs sum_z")
+        return (self._in(sum_ym, ny * nm + 1, "sum_ym"), self._in(sum_z, nm * (nz + 1), "sum_z"),
+                self._in(sum_zs, nm * (nz + 1), "sum_zs"), self._in(sum_zp, nm * (nz + 1), "sum_zp"),
+                self._in(ratio, ny * nm, "ratio"))
 
     def photon_pt_tables(self, gtot, stream=None):
         _check(load().upc_photon_pt_tables(self._ctx, gtot, _stream(stream)), "upc_photon_pt_tables")
 
     def photon_pt(self, E, seed=1, stream=None):
         out = self._empty(E.numel())
-        _check(load().upc_photon_pt(self._ctx, _dptr(E), E.numel(), seed, _dptr(out), _stream(stream)),
+        _check(load().upc_photon_pt(self._ctx, self._in(E, name="E"), E.numel(), seed, _dptr(out), _stream(stream)),
                "upc_photon_pt")
         return out
 
@@ -389,20 +433,22 @@ class Context:
         """Dilepton events: (p [n,2,4] (px,py,pz,E), pdg [n,2], pass [n])."""
         import torch
         o = opts if opts is not None else EventOpts(0, 0, 0.0, 0, 0.0, 0.0)
-        p = torch.empty((n, 2, 4), dtype=torch.float64, device="cuda")
-        pdg = torch.empty((n, 2), dtype=torch.int32, device="cuda")
-        ok = torch.empty(n, dtype=torch.int32, device="cuda")
-        _check(load().upc_generate_pairs(self._ctx, _dptr(sum_ym), _dptr(sum_z), _dptr(sum_zs), _dptr(sum_zp),
-                                         _dptr(ratio), C.byref(o), seed, offset, n, _dptr(p), _dptr(pdg),
-                                         _dptr(ok), _stream(stream)), "upc_generate_pairs")
+        dev = f"cuda:{self.device}"
+        tables = self._tables(sum_ym, sum_z, sum_zs, sum_zp, ratio)
+        p = torch.empty((n, 2, 4), dtype=torch.float64, device=dev)
+        pdg = torch.empty((n, 2), dtype=torch.int32, device=dev)
+        ok = torch.empty(n, dtype=torch.int32, device=dev)
+        _check(load().upc_generate_pairs(self._ctx, *tables, C.byref(o), seed, offset, n, _dptr(p),
+                                         _dptr(pdg, torch.int32), _dptr(ok, torch.int32), _stream(stream)),
+               "upc_generate_pairs")
         return p, pdg, ok
 
     def sample_events(self, sum_ym, sum_z=None, sum_zs=None, sum_zp=None, ratio=None, n=1000,
                       seed=12345, offset=0, stream=None):
+        tables = self._tables(sum_ym, sum_z, sum_zs, sum_zp, ratio)
         y, m, z = self._empty(n), self._empty(n), self._empty(n)
-        _check(load().upc_sample_events(self._ctx, _dptr(sum_ym), _dptr(sum_z), _dptr(sum_zs),
-                                        _dptr(sum_zp), _dptr(ratio), seed, offset, n, _dptr(y), _dptr(m),
-                                        _dptr(z), _stream(stream)), "upc_sample_events")
+        _check(load().upc_sample_events(self._ctx, *tables, seed, offset, n, _dptr(y), _dptr(m), _dptr(z),
+                                        _stream(stream)), "upc_sample_events")
         return y, m, z
 
 
@@ -490,6 +536,14 @@ class Batch:
 
     def set_tune(self, qags_blocks_per_cu, lumi_waves):
         _check(load().upc_batch_set_tune(self._b, qags_blocks_per_cu, lumi_waves), "upc_batch_set_tune")
+
+    def job_failures(self, stream=None):
+        """Per job of the last run: flux integrals that ended with a QAGS error
+        or a form-factor spline domain error (the reference aborts there)."""
+        n = len(self.configs)
+        out = (_ll * n)()
+        _check(load().upc_batch_job_failures(self._b, _stream(stream), out), "upc_batch_job_failures")
+        return list(out)
 
     def stats(self, stream=None) -> BatchStats:
         st = BatchStats()

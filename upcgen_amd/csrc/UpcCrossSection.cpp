@@ -6,6 +6,7 @@
 
 #include <fcntl.h>
 #include <hip/hip_runtime.h>
+#include <sys/file.h>
 #include <sys/stat.h>
 #include <unistd.h>
 
@@ -20,10 +21,29 @@
 
 namespace
 {
+// The luminosity lock this process holds while it computes the grid: the
+// lock file (created, or taken over when stale) and an exclusive flock on it.
+// Fatal exits release it first, so that later runs do not wait for it.
+struct HeldLock {
+  std::string path;
+  int fd = -1;
+};
+HeldLock g_lock;
+
+void releaseHeldLock()
+{
+  if (g_lock.fd < 0) return;
+  if (std::remove(g_lock.path.c_str()) != 0)
+    UPC_LOG_WARNING << "Lock file " << g_lock.path << " was not properly removed!";
+  close(g_lock.fd);  // drops the flock
+  g_lock = HeldLock{};
+}
+
 void hipCheck(hipError_t e, const char* what)
 {
   if (e != hipSuccess) {
     UPC_LOG_FATAL << what << ": " << hipGetErrorString(e);
+    releaseHeldLock();
     std::_Exit(-1);
   }
 }
@@ -32,7 +52,54 @@ void abiCheck(int rc, const char* what)
 {
   if (rc != 0) {
     UPC_LOG_FATAL << what << " failed: " << upc_last_error();
+    releaseHeldLock();
     std::_Exit(-1);
+  }
+}
+
+bool sameFile(int fd, const std::string& path)
+{
+  struct stat a {}, b {};
+  return fstat(fd, &a) == 0 && stat(path.c_str(), &b) == 0 && a.st_dev == b.st_dev && a.st_ino == b.st_ino;
+}
+
+// Waits for and takes the luminosity lock at `path` (src/UpcCrossSection.cpp:
+// 465-478: the lock file's existence means another generator is computing).
+// The holder keeps an exclusive flock on the file, so a lock is stale only
+// when nobody holds its flock (the holder died, or it was left by a process
+// that does not take one) and it is older than timeout_s (< 0: never).  A
+// stale lock is taken over by acquiring its flock and refreshing its time, so
+// of several waiters exactly one takes it; a holder that ends removes the file
+// before dropping the flock, and a waiter that got the flock of a file no
+// longer at `path` starts over.
+void acquireLock(const std::string& path, int timeout_s)
+{
+  while (true) {
+    int fd = open(path.c_str(), O_CREAT | O_EXCL | O_RDWR, 0644);
+    if (fd >= 0) {
+      if (flock(fd, LOCK_EX) == 0 && sameFile(fd, path)) {
+        g_lock = HeldLock{path, fd};
+        return;
+      }
+      close(fd);
+      continue;
+    }
+    fd = open(path.c_str(), O_RDWR);
+    if (fd < 0) continue;  // removed meanwhile: try to create it again
+    struct stat st {};
+    const bool old = timeout_s >= 0 && fstat(fd, &st) == 0 &&
+                     std::difftime(std::time(nullptr), st.st_mtime) > timeout_s;
+    if (old && flock(fd, LOCK_EX | LOCK_NB) == 0) {
+      if (sameFile(fd, path)) {
+        futimens(fd, nullptr);
+        UPC_LOG_WARNING << "Lock file " << path << " is older than " << timeout_s
+                        << " s and nobody holds it: taking it over";
+        g_lock = HeldLock{path, fd};
+        return;
+      }
+    }
+    close(fd);
+    sleep(1);
   }
 }
 
@@ -235,6 +302,8 @@ void UpcCrossSection::evalPoints(int what, const double* x, const double* y, int
       abiCheck(upc_breakup_prob(ctx_, dbuf_, n, (int)y[0], 0, o, nullptr), "upc_breakup_prob");
       break;
     case 4: abiCheck(upc_lumi_points(ctx_, dbuf_, dbuf_ + 1, n, o, nullptr, nullptr, nullptr), "upc_lumi_points"); break;
+    case 6: abiCheck(upc_photon_flux(ctx_, dbuf_, dbuf_ + 1, n, o, nullptr), "upc_photon_flux"); break;
+    case 7: abiCheck(upc_photon_pt(ctx_, dbuf_, n, (unsigned long long)y[0], o, nullptr), "upc_photon_pt"); break;
     default:
       abiCheck(upc_lumi_points(ctx_, dbuf_, dbuf_ + 1, n, nullptr, o, o + 1, nullptr), "upc_lumi_points");
       break;
@@ -286,6 +355,85 @@ double UpcCrossSection::calcTwoPhotonLumi(double M, double Y)
   double o;
   evalPoints(4, &M, &Y, 1, &o);
   return o;
+}
+
+double UpcCrossSection::calcPhotonFlux(double M, double Y)
+{
+  double o;
+  evalPoints(6, &M, &Y, 1, &o);
+  return o;
+}
+
+void UpcCrossSection::calcNucCrossSectionY(std::vector<std::vector<double>>& crossSectionY,
+                                           std::vector<std::vector<double>>& csYRatio, double& totCS)
+{
+  UPC_LOG_INFO << "Calculating nuclear cross section...";
+  ensureContext();
+  const double dy = (ymax - ymin) / ny;
+  std::vector<double> h(4 * (size_t)ny);  // sigma(y), sigma(-y), cs, ratio
+  for (int iy = 0; iy < ny; ++iy) {
+    const double y = ymin + dy * iy;
+    h[iy] = elemProcess->calcCrossSectionY(y);
+    h[ny + iy] = elemProcess->calcCrossSectionY(-y);
+  }
+  double* d = nullptr;
+  hipCheck(hipMalloc(&d, sizeof(double) * h.size()), "hipMalloc");
+  hipCheck(hipMemcpy(d, h.data(), sizeof(double) * 2 * ny, hipMemcpyHostToDevice), "hipMemcpy");
+  abiCheck(upc_nuc_cs_y(ctx_, elemProcess->mPart, d, d + ny, d + 2 * ny, d + 3 * ny, &totCS, nullptr),
+           "upc_nuc_cs_y");
+  hipCheck(hipMemcpy(h.data(), d, sizeof(double) * h.size(), hipMemcpyDeviceToHost), "hipMemcpy");
+  (void)hipFree(d);
+  crossSectionY.resize(ny);
+  csYRatio.resize(ny);
+  for (int iy = 0; iy < ny; ++iy) {
+    crossSectionY[iy].resize(1);
+    csYRatio[iy].resize(1);
+    crossSectionY[iy][0] = h[2 * ny + iy];
+    csYRatio[iy][0] = h[3 * ny + iy];
+  }
+}
+
+double UpcCrossSection::uniform01()
+{
+  // splitmix64 (Steele, Lea, Flood 2014): 53 random bits per draw
+  unsigned long long z = (rngState_ += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  return (double)(z >> 11) * 0x1.0p-53;
+}
+
+double UpcCrossSection::getPhotonPt(double ePhot)
+{
+  ensureContext();
+  if (!ptTables_) {
+    abiCheck(upc_photon_pt_tables(ctx_, gtot, nullptr), "upc_photon_pt_tables");
+    ptTables_ = true;
+  }
+  const double seed = (double)(++ptDraws_);  // one Philox stream per draw
+  double o;
+  evalPoints(7, &ePhot, &seed, 1, &o);
+  return o;
+}
+
+void UpcCrossSection::getPairMomentum(double mPair, double yPair, UpcLorentzVector& pPair)
+{
+  if (!useNonzeroGamPt) {
+    const double mtPair = mPair;  // pair pt = 0
+    pPair.SetPxPyPzE(0., 0., mtPair * std::sinh(yPair), mtPair * std::cosh(yPair));
+    return;
+  }
+  const double k1 = mPair / 2 * std::exp(yPair);
+  const double k2 = mPair / 2 * std::exp(-yPair);
+  const double angle1 = 2 * M_PI * uniform01();
+  const double angle2 = 2 * M_PI * uniform01();
+  const double pt1 = getPhotonPt(k1);
+  const double pt2 = getPhotonPt(k2);
+  const double px = pt1 * std::cos(angle1) + pt2 * std::cos(angle2);
+  const double py = pt1 * std::sin(angle1) + pt2 * std::sin(angle2);
+  const double pt = std::sqrt(px * px + py * py);
+  const double mtPair = std::sqrt(mPair * mPair + pt * pt);
+  pPair.SetPxPyPzE(px, py, mtPair * std::sinh(yPair), mtPair * std::cosh(yPair));
 }
 
 void UpcCrossSection::calcTwoPhotonLumiPol(double& ns, double& np, double M, double Y)
@@ -356,7 +504,8 @@ bool UpcCrossSection::readLumiCache(std::vector<double>& l0, std::vector<double>
 
 void UpcCrossSection::writeLumiCache(const std::vector<double>& l0, const std::vector<double>& l1) const
 {
-  const std::string tmp = lumiFileName() + ".tmp";
+  // a file of this process's own, renamed into place when complete
+  const std::string tmp = lumiFileName() + ".tmp." + std::to_string((long)getpid());
   {
     std::ofstream f(tmp, std::ios::binary | std::ios::trunc);
     const LumiCacheHeader h = makeHeader(config(), lumiForm);
@@ -365,33 +514,22 @@ void UpcCrossSection::writeLumiCache(const std::vector<double>& l0, const std::v
     if (usePolarizedCS) f.write(reinterpret_cast<const char*>(l1.data()), sizeof(double) * l1.size());
     if (!f) {
       UPC_LOG_WARNING << "Could not write " << tmp;
+      std::remove(tmp.c_str());
       return;
     }
   }
-  std::rename(tmp.c_str(), lumiFileName().c_str());
+  if (std::rename(tmp.c_str(), lumiFileName().c_str()) != 0) {
+    UPC_LOG_WARNING << "Could not rename " << tmp << " to " << lumiFileName();
+    std::remove(tmp.c_str());
+  }
 }
 
 void UpcCrossSection::prepareTwoPhotonLumi()
 {
   ensureContext();
   // wait here if another generator is working on this (src/UpcCrossSection.cpp:465-478),
-  // taking over a lock older than lockTimeoutSec
-  const std::string fnlock = lockFileName();
-  while (true) {
-    const int fd = open(fnlock.c_str(), O_CREAT | O_EXCL | O_WRONLY, 0644);
-    if (fd >= 0) {
-      close(fd);
-      break;
-    }
-    struct stat st {};
-    if (lockTimeoutSec >= 0 && stat(fnlock.c_str(), &st) == 0 &&
-        std::difftime(std::time(nullptr), st.st_mtime) > lockTimeoutSec) {
-      UPC_LOG_WARNING << "Lock file " << fnlock << " is older than " << lockTimeoutSec << " s: taking it over";
-      std::remove(fnlock.c_str());
-      continue;
-    }
-    sleep(1);
-  }
+  // taking over a stale lock (see acquireLock)
+  acquireLock(lockFileName(), lockTimeoutSec);
 
   std::vector<double> l0, l1;
   if (readLumiCache(l0, l1, strictCacheKey)) {
@@ -414,13 +552,27 @@ void UpcCrossSection::prepareTwoPhotonLumi()
     (void)hipFree(d);
     upc_info inf{};
     upc_ctx_info(ctx_, &inf);
-    if (inf.qags_bad)
-      UPC_LOG_WARNING << inf.qags_bad << " form-factor flux integrals ended with a QAGS error"
-                      << " (the reference aborts there)";
+    if (inf.qags_bad) {
+      // The reference process stops at the first of these: GSL's default
+      // error handler aborts on a QAGS error (GSL_EROUND, ...) and on a
+      // form-factor spline evaluation past the table's last knot (GSL_EDOM),
+      // whose NaN then fills the bins that use the flux row.  Here those
+      // bins are emptied so that the cross sections and samplers stay usable.
+      size_t nnan = 0;
+      for (std::vector<double>* v : {&l0, &l1})
+        for (double& x : *v)
+          if (std::isnan(x)) {
+            x = 0.;
+            ++nnan;
+          }
+      UPC_LOG_ERROR << inf.qags_bad << " form-factor flux integrals ended with a QAGS error or a form-factor"
+                    << " spline domain error (GSL would abort the reference here); " << nnan
+                    << " luminosity bins with NaN are set to 0";
+    }
     writeLumiCache(l0, l1);
     UPC_LOG_INFO << "Two-photon luminosity was written to " << lumiFileName();
   }
-  if (std::remove(fnlock.c_str()) != 0) UPC_LOG_WARNING << "Lock file " << fnlock << " was not properly removed!";
+  releaseHeldLock();
 }
 
 void UpcCrossSection::calcNucCrossSectionYM(std::vector<std::vector<double>>& crossSectionYM,

@@ -27,6 +27,35 @@ constexpr double mMu{0.1056583745};
 constexpr double mTau{1.77686};
 }  // namespace phys_consts
 
+// Minimal four-vector with the accessors the reference's uses of
+// TLorentzVector need (SetPxPyPzE, Px..E, Pt, M, Eta).
+struct UpcLorentzVector {
+  double px{0}, py{0}, pz{0}, e{0};
+  void SetPxPyPzE(double x, double y, double z, double t)
+  {
+    px = x;
+    py = y;
+    pz = z;
+    e = t;
+  }
+  double Px() const { return px; }
+  double Py() const { return py; }
+  double Pz() const { return pz; }
+  double E() const { return e; }
+  double Pt() const { return std::sqrt(px * px + py * py); }
+  double Mag2() const { return e * e - px * px - py * py - pz * pz; }
+  double M() const
+  {
+    const double m2 = Mag2();
+    return m2 < 0 ? -std::sqrt(-m2) : std::sqrt(m2);
+  }
+  double Eta() const
+  {
+    const double pt = Pt();
+    return pt > 0 ? std::asinh(pz / pt) : (pz >= 0 ? 10e10 : -10e10);
+  }
+};
+
 class UpcCrossSection
 {
  public:
@@ -129,7 +158,20 @@ class UpcCrossSection
                           double mmin, double mmax, int nm, int flag);
   void calcNucCrossSectionYM(std::vector<std::vector<double>>& crossSectionYM,
                              std::vector<std::vector<double>>& polCSRatio, double& totCS);
+  // vector-meson path: photon flux at k = M/2 e^Y weighted by P(b) G_AA(b),
+  // and sigma(y) = flux(m, y) sigma_el(y) + flux(m, -y) sigma_el(-y) with the
+  // elementary sigma_el(y) of elemProcess (src/UpcCrossSection.cpp:699-748)
+  double calcPhotonFlux(double M, double Y);
+  void calcNucCrossSectionY(std::vector<std::vector<double>>& crossSectionY,
+                            std::vector<std::vector<double>>& csYRatio, double& totCS);
   double calcBreakupProb(double b, int mode);
+
+  // pair momentum with non-zero photon pt (src/UpcCrossSection.cpp:1021-1074):
+  // photon pt drawn from the per-energy form-factor distribution on the GPU,
+  // azimuths from this object's own generator (the reference's gRandom)
+  double getPhotonPt(double ePhot);
+  void getPairMomentum(double mPair, double yPair, UpcLorentzVector& pPair);
+  void setRandomSeed(unsigned long long seed) { rngState_ = seed; }
 
   void prepareGAA();
   void prepareBreakupProb();
@@ -148,6 +190,10 @@ class UpcCrossSection
   bool readLumiCache(std::vector<double>& l0, std::vector<double>& l1, bool requireKey) const;
   void writeLumiCache(const std::vector<double>& l0, const std::vector<double>& l1) const;
   inline static UpcCrossSection* current_{nullptr};
+  bool ptTables_{false};
+  unsigned long long ptDraws_{0};     // photon-pt draws so far (Philox stream per draw)
+  unsigned long long rngState_{12345};
+  double uniform01();                 // splitmix64 stream for the azimuths
 };
 
 template <typename ArrayType>

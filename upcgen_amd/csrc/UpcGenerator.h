@@ -15,27 +15,6 @@
 #include "UpcCrossSection.h"
 #include "UpcSampler.h"
 
-// Minimal four-vector with the accessors the reference's TLorentzVector use needs.
-struct UpcLorentzVector {
-  double px{0}, py{0}, pz{0}, e{0};
-  double Px() const { return px; }
-  double Py() const { return py; }
-  double Pz() const { return pz; }
-  double E() const { return e; }
-  double Pt() const { return std::sqrt(px * px + py * py); }
-  double Mag2() const { return e * e - px * px - py * py - pz * pz; }
-  double M() const
-  {
-    const double m2 = Mag2();
-    return m2 < 0 ? -std::sqrt(-m2) : std::sqrt(m2);
-  }
-  double Eta() const
-  {
-    const double pt = Pt();
-    return pt > 0 ? std::asinh(pz / pt) : (pz >= 0 ? 10e10 : -10e10);
-  }
-};
-
 class UpcGenerator
 {
  public:

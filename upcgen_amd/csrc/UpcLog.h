@@ -50,6 +50,7 @@ class Line
 }  // namespace upclog
 
 #define UPC_LOG_FATAL upclog::Line(upclog::FATAL)
+#define UPC_LOG_ERROR upclog::Line(upclog::ERROR_)
 #define UPC_LOG_WARNING upclog::Line(upclog::WARNING)
 #define UPC_LOG_INFO upclog::Line(upclog::INFO)
 #define UPC_LOG_DEBUG upclog::Line(upclog::DEBUG)

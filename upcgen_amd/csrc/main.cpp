@@ -1,66 +1,101 @@
-// upcgen command line (main.cpp:30-116 of the reference): -h, -debug N,
-// -nthreads N, -parfile FILE; plus -form fp64|mfma|surrogate, -lumidir DIR
-// and -printpars (print the parsed parameters and exit, no GPU needed).
-#include <algorithm>
+// upcgen command line.  The flags of the reference's executable (main.cpp:
+// 60-116 of the upcgen tree: -h, -debug N, -nthreads N, -parfile FILE) keep
+// their names and meaning; -form, -lumidir, -hepmc, -locktimeout and
+// -printpars are this build's.  Options are read in one pass over argv
+// against a table: every option is "-name" or "-name value", a later
+// occurrence overrides an earlier one, and an unknown option or a missing
+// value is reported (the reference ignores both silently) without stopping.
 #include <cstdio>
+#include <cstdlib>
+#include <map>
 #include <string>
-#include <vector>
 
 #include "UpcGenerator.h"
 #include "UpcLog.h"
 
-class InputParser
-{
- public:
-  InputParser(int& argc, char** argv)
-  {
-    for (int i = 1; i < argc; ++i) tokens.emplace_back(argv[i]);
-  }
-  const std::string& getCmdOption(const std::string& option) const
-  {
-    auto itr = std::find(tokens.begin(), tokens.end(), option);
-    if (itr != tokens.end() && ++itr != tokens.end()) return *itr;
-    static const std::string empty;
-    return empty;
-  }
-  bool cmdOptionExists(const std::string& option) const
-  {
-    return std::find(tokens.begin(), tokens.end(), option) != tokens.end();
-  }
+namespace {
 
- private:
-  std::vector<std::string> tokens;
+struct OptionSpec {
+  const char* name;
+  bool takes_value;
+  const char* help;
 };
+
+constexpr OptionSpec kOptions[] = {
+    {"-h", false, "help message"},
+    {"-debug", true, "debug level: 0 = no debug messages (default), 1 = basic debug messages,\n"
+                     "             2 = intermediate results and events"},
+    {"-nthreads", true, "accepted for compatibility (the luminosity runs on the GPU)"},
+    {"-parfile", true, "name of the input parameter file (default parameters.in)"},
+    {"-form", true, "luminosity form: fp64 (default), mfma, surrogate"},
+    {"-lumidir", true, "directory of the cached luminosity grid and its lock file"},
+    {"-hepmc", true, "HepMC output file name (default events.hepmc)"},
+    {"-locktimeout", true, "seconds after which a luminosity lock file is taken over"},
+    {"-printpars", false, "print the parameters read and exit"},
+};
+
+const OptionSpec* findOption(const std::string& arg)
+{
+  for (const OptionSpec& o : kOptions)
+    if (arg == o.name) return &o;
+  return nullptr;
+}
+
+// name -> value ("" for flags); unknown options and missing values are logged
+std::map<std::string, std::string> parseArgs(int argc, char** argv)
+{
+  std::map<std::string, std::string> seen;
+  for (int i = 1; i < argc; ++i) {
+    const std::string arg = argv[i];
+    const OptionSpec* o = findOption(arg);
+    if (!o) {
+      UPC_LOG_WARNING << "Ignoring unknown command-line argument " << arg;
+      continue;
+    }
+    if (!o->takes_value) {
+      seen[arg] = "";
+    } else if (i + 1 < argc) {
+      seen[arg] = argv[++i];
+    } else {
+      UPC_LOG_WARNING << "Option " << arg << " needs a value; ignored";
+    }
+  }
+  return seen;
+}
+
+void printHelp()
+{
+  std::printf("Available options:\n");
+  for (const OptionSpec& o : kOptions) std::printf("%-12s -- %s\n", o.name, o.help);
+  std::fflush(stdout);
+}
+
+int toInt(const std::map<std::string, std::string>& opts, const char* name, int fallback)
+{
+  const auto it = opts.find(name);
+  if (it == opts.end()) return fallback;
+  char* end = nullptr;
+  const long v = std::strtol(it->second.c_str(), &end, 10);
+  if (end == it->second.c_str() || *end != '\0') {
+    UPC_LOG_WARNING << "Option " << name << " expects an integer, got '" << it->second << "'; using " << fallback;
+    return fallback;
+  }
+  return (int)v;
+}
+
+}  // namespace
 
 int main(int argc, char** argv)
 {
-  InputParser input(argc, argv);
-  int debugLevel = 0;
-  int numThreads = 1;
-  std::string parFileName("parameters.in");
-  if (input.cmdOptionExists("-h")) {
-    std::printf(
-        "Available options:\n"
-        "-h         -- help message\n"
-        "-debug     -- set debug level: 0=no debug messages (default),\n"
-        "                               1=basic debug messages,\n"
-        "                               2=print out intermediate calculation results and events info\n"
-        "-nthreads  -- accepted for compatibility (the luminosity runs on the GPU)\n"
-        "-parfile   -- name of input parameter file\n"
-        "-form      -- luminosity form: fp64 (default), mfma, surrogate\n"
-        "-lumidir   -- directory of the cached luminosity grid and its lock file\n"
-        "-hepmc     -- HepMC output file name (default events.hepmc)\n"
-        "-locktimeout -- seconds after which a luminosity lock file is taken over\n"
-        "-printpars -- print the parameters read and exit\n");
-    std::fflush(stdout);
+  const auto opts = parseArgs(argc, argv);
+  if (opts.count("-h")) {
+    printHelp();
     std::_Exit(0);
   }
-  const auto dbg = input.getCmdOption("-debug");
-  if (!dbg.empty()) debugLevel = std::stoi(dbg);
-  const auto nth = input.getCmdOption("-nthreads");
-  if (!nth.empty()) numThreads = std::stoi(nth);
-  const auto pf = input.getCmdOption("-parfile");
-  if (!pf.empty()) parFileName = pf;
+  const int debugLevel = toInt(opts, "-debug", 0);
+  const int numThreads = toInt(opts, "-nthreads", 1);
+  const auto pf = opts.find("-parfile");
+  const std::string parFileName = pf != opts.end() ? pf->second : "parameters.in";
 
   UPC_LOG_INFO << "Configuring the generator...";
   auto* upcGenerator = new UpcGenerator();
@@ -68,16 +103,15 @@ int main(int argc, char** argv)
   upcGenerator->setNumThreads(numThreads);
   upcGenerator->setParFile(parFileName);
   upcGenerator->configGeneratorFromFile();
-  const auto form = input.getCmdOption("-form");
-  if (form == "mfma") upcGenerator->setLumiForm(UPC_FORM_MFMA);
-  if (form == "surrogate") upcGenerator->setLumiForm(UPC_FORM_SURROGATE);
-  const auto dir = input.getCmdOption("-lumidir");
-  if (!dir.empty()) upcGenerator->setLumiFileDirectory(dir);
-  const auto lt = input.getCmdOption("-locktimeout");
-  if (!lt.empty()) upcGenerator->crossSection()->lockTimeoutSec = std::stoi(lt);
-  const auto hm = input.getCmdOption("-hepmc");
-  if (!hm.empty()) upcGenerator->hepmcFileName = hm;
-  if (input.cmdOptionExists("-printpars")) {
+  if (const auto f = opts.find("-form"); f != opts.end()) {
+    if (f->second == "mfma") upcGenerator->setLumiForm(UPC_FORM_MFMA);
+    else if (f->second == "surrogate") upcGenerator->setLumiForm(UPC_FORM_SURROGATE);
+    else if (f->second != "fp64") UPC_LOG_WARNING << "Unknown -form " << f->second << "; using fp64";
+  }
+  if (const auto d = opts.find("-lumidir"); d != opts.end()) upcGenerator->setLumiFileDirectory(d->second);
+  if (opts.count("-locktimeout")) upcGenerator->crossSection()->lockTimeoutSec = toInt(opts, "-locktimeout", 3600);
+  if (const auto h = opts.find("-hepmc"); h != opts.end()) upcGenerator->hepmcFileName = h->second;
+  if (opts.count("-printpars")) {
     upcGenerator->printParameters();
     delete upcGenerator;
     return 0;

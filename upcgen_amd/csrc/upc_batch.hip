@@ -95,12 +95,14 @@ struct BatchTask {
   const BatchJob* __restrict__ jobs;
   const int* __restrict__ row0;
   int njobs;
+  unsigned long long* __restrict__ job_failed;  // [njobs] integrals with ier != 0
   __device__ void get(int t, FluxTask* ft) const
   {
     const int id = tasks[t];
     const double2 v = bk[t];
     const int r = id >> 7, j = id & 127;
-    const BatchJob& J = jobs[find_job(row0, njobs, r)];
+    const int jb = find_job(row0, njobs, r);
+    const BatchJob& J = jobs[jb];
     ft->b = v.x;
     ft->k = v.y;
     ft->out = (long)r * kNB1 + j;
@@ -108,7 +110,9 @@ struct BatchTask {
     ft->g1 = J.v.fp.g1;
     ft->factor = J.v.fp.factor;
     ft->A = J.v.fp.A;
+    ft->slot = jb;
   }
+  __device__ void failed(const FluxTask& ft) const { atomicAdd(&job_failed[ft.slot], 1ull); }
 };
 
 // Contraction job resolver: blk0[j] = first block of job j for this wave count.
@@ -152,6 +156,7 @@ struct upc_batch {
   double4* d_first = nullptr;  // first QK21 pass of each task (k_flux_first)
   int* d_overflow = nullptr;
   unsigned long long* d_counters = nullptr;  // [0] evals [1] bad [2] overflow n [4] tasks [5] cursor
+  unsigned long long* d_job_failed = nullptr;  // [njobs] QAGS integrals with ier != 0 per job
   QagsBigArrays* d_big = nullptr;
   int nbig = 256;
   int qags_choice = 2, lumi_choice = 1;  // indices into the candidate lists
@@ -180,7 +185,7 @@ void free_batch(upc_batch* b)
   if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
   void* ptrs[] = {b->d_jobs, b->d_pt0, b->d_row0, b->d_blk0[0], b->d_blk0[1], b->d_blk0[2], b->d_pts,
                   b->d_row_k, b->d_row_logd, b->d_row_cnt, b->d_row_off, b->d_part, b->d_flux, b->d_tasks,
-                  b->d_task_bk, b->d_first, b->d_overflow, b->d_counters, b->d_big};
+                  b->d_task_bk, b->d_first, b->d_overflow, b->d_counters, b->d_job_failed, b->d_big};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (upc_ctx* c : b->ctx) upc_ctx_destroy(c);
@@ -192,6 +197,7 @@ int enqueue(upc_batch* b, hipStream_t s)
 {
   const int nj = (int)b->jobs.size();
   hipLaunchKernelGGL(k_batch_zero, dim3(1), dim3(64), 0, s, b->d_counters, 8);
+  hipLaunchKernelGGL(k_batch_zero, dim3(nblk(nj, 256)), dim3(256), 0, s, b->d_job_failed, nj);
   hipLaunchKernelGGL(k_batch_points, dim3(nblk(b->npts, 256)), dim3(256), 0, s, b->d_jobs, b->d_pt0, nj,
                      b->npts, b->d_pts);
   hipLaunchKernelGGL(k_batch_rows, dim3(nblk(b->nrows, 256)), dim3(256), 0, s, b->d_jobs, b->d_row0, nj,
@@ -203,7 +209,7 @@ int enqueue(upc_batch* b, hipStream_t s)
   hipLaunchKernelGGL(k_batch_point_rows, dim3(nblk(b->nrows * kNB1, 256)), dim3(256), 0, s, b->d_jobs, b->d_row0,
                      nj, b->nrows, b->d_row_k, b->d_row_logd, b->d_row_cnt, b->d_row_off, b->d_flux, b->d_tasks,
                      b->d_task_bk);
-  BatchTask ts{b->d_tasks, b->d_task_bk, b->d_jobs, b->d_row0, nj};
+  BatchTask ts{b->d_tasks, b->d_task_bk, b->d_jobs, b->d_row0, nj, b->d_job_failed};
   const int qblocks = b->ncu * kQagsBlockChoices[b->qags_choice];
   hipLaunchKernelGGL((k_flux_first<BatchTask>), dim3(b->ncu * kQagsBlockChoices[2]), dim3(256), 0, s, ts,
                      b->d_counters + 4, b->d_first);
@@ -285,6 +291,7 @@ int upc_batch_create(const upc_config* cfgs, int njobs, void* stream, upc_batch*
     BA(b->d_first, nr * kNB1);
     BA(b->d_overflow, nr * kNB1);
     BA(b->d_counters, 8);
+    BA(b->d_job_failed, njobs);
     BA(b->d_big, b->nbig);
     return 0;
   };
@@ -404,7 +411,7 @@ int upc_batch_autotune(upc_batch* b, void* stream, upc_tune* h_out)
   UPC_HIP(hipEventCreate(&e0));
   UPC_HIP(hipEventCreate(&e1));
   const int nj = (int)b->jobs.size();
-  BatchTask ts{b->d_tasks, b->d_task_bk, b->d_jobs, b->d_row0, nj};
+  BatchTask ts{b->d_tasks, b->d_task_bk, b->d_jobs, b->d_row0, nj, b->d_job_failed};
   for (int i = 0; i < 3; ++i) {
     // QAGS alone (the first passes of the warm run stay valid: they depend on
     // the task list only): clear the counters except the task total, then time
@@ -460,6 +467,15 @@ int upc_batch_set_tune(upc_batch* b, int qags_blocks_per_cu, int lumi_waves)
   if (qi < 0 || li < 0) return set_error("unsupported launch shape"), UPC_EINVAL;
   b->qags_choice = qi;
   b->lumi_choice = li;
+  return 0;
+}
+
+int upc_batch_job_failures(upc_batch* b, void* stream, long long* h_failed)
+{
+  if (!b || !h_failed) return set_error("null pointer"), UPC_EINVAL;
+  hipStream_t s = (hipStream_t)stream;
+  UPC_HIP(hipMemcpyAsync(h_failed, b->d_job_failed, sizeof(long long) * b->jobs.size(), hipMemcpyDeviceToHost, s));
+  UPC_HIP(hipStreamSynchronize(s));
   return 0;
 }
 

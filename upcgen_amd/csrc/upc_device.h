@@ -40,6 +40,9 @@ constexpr double kQ2min = 1e-9;
 constexpr double kQ2max = 2.;
 constexpr int kNQ2 = 1000000;
 constexpr double kDQ2 = (kQ2max - kQ2min) / kNQ2;
+// Last knot of the table, Q2min + (nQ2 - 1) dQ2 with each operation rounded
+// (as the reference's table loop computes it): the end of the spline's range.
+constexpr double kQ2Last = kQ2min + (double)(kNQ2 - 1) * kDQ2;
 
 // src/UpcCrossSection.cpp:418-421: breakup table grid
 constexpr double kBkMin = 1e-6;
@@ -488,6 +491,66 @@ __device__ __forceinline__ QK21Out qk21_lds(const F& f, double a, double b, doub
 #pragma unroll
   for (int j = 0; j < 10; ++j)
     rasc = rasc + kWgk[j] * (fabs(fv[(2 * j) * stride] - reskh) + fabs(fv[(2 * j + 1) * stride] - reskh));
+  QK21Out o;
+  o.result = resk * hlgth;
+  rabs = rabs * dhlgth;
+  rasc = rasc * dhlgth;
+  double err = fabs((resk - resg) * hlgth);
+  if (rasc != 0.0 && err != 0.0) {
+    const double t15 = 200.0 * err / rasc;
+    const double sc = t15 * sqrt(t15);
+    err = rasc * (sc < 1.0 ? sc : 1.0);
+  }
+  if (rabs > uflow / (50.0 * epmach)) {
+    const double m = (epmach * 50.0) * rabs;
+    if (m > err) err = m;
+  }
+  o.abserr = err;
+  o.resabs = rabs;
+  o.resasc = rasc;
+  return o;
+}
+
+// dqk21 as the rolled loop of qk21_lds with the 20 off-centre values in a
+// local array indexed by the (wave-uniform) node counter.
+template <class F>
+__device__ __forceinline__ QK21Out qk21_rolled(const F& f, double a, double b)
+{
+  constexpr double epmach = DBL_EPSILON, uflow = DBL_MIN;
+  double fv[20];
+  const double centr = 0.5 * (a + b);
+  const double hlgth = 0.5 * (b - a);
+  const double dhlgth = fabs(hlgth);
+  double resg = 0.0, resk = 0.0, rabs = 0.0, fc = 0.0, f1 = 0.0;
+#pragma unroll 1
+  for (int e = 0; e < 21; ++e) {
+    const bool gauss = e <= 10;
+    const int p = gauss ? (e - 1) >> 1 : (e - 11) >> 1;
+    const int side = gauss ? (e - 1) & 1 : (e - 11) & 1;
+    const int xi = e == 0 ? 10 : (gauss ? 2 * p + 1 : 2 * p);
+    const double absc = hlgth * kXgk[xi];
+    const double x = e == 0 ? centr : (side ? centr + absc : centr - absc);
+    const double fx = f(x);
+    if (e == 0) {
+      fc = fx;
+      resk = kWgk[10] * fc;
+      rabs = fabs(resk);
+    } else if (side == 0) {
+      f1 = fx;
+    } else {
+      const double f2 = fx;
+      fv[2 * xi] = f1;
+      fv[2 * xi + 1] = f2;
+      const double fsum = f1 + f2;
+      if (gauss) resg = resg + kWg[p] * fsum;
+      resk = resk + kWgk[xi] * fsum;
+      rabs = rabs + kWgk[xi] * (fabs(f1) + fabs(f2));
+    }
+  }
+  const double reskh = resk * 0.5;
+  double rasc = kWgk[10] * fabs(fc - reskh);
+#pragma unroll
+  for (int j = 0; j < 10; ++j) rasc = rasc + kWgk[j] * (fabs(fv[2 * j] - reskh) + fabs(fv[2 * j + 1] - reskh));
   QK21Out o;
   o.result = resk * hlgth;
   rabs = rabs * dhlgth;

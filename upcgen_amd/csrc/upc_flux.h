@@ -147,6 +147,13 @@ __device__ __forceinline__ void row_setup(const double* __restrict__ pts, int np
 
 // fluxFormIntegrand (src/UpcCrossSection.cpp:181-191):
 //   q^2 F(min(t, Q2max - dQ2)) / t * J1(b q / hc),  t = q^2 + (k / g1)^2
+//
+// The table's last knot is Q2min + (nQ2 - 1) dQ2 = 2 - 2e-6 (to rounding),
+// so for t in (last knot, Q2max) the reference's gsl_spline_eval is out of
+// range: GSL raises GSL_EDOM (its default handler aborts) and returns NaN.
+// The integrand returns that NaN, so the QAGS run continues exactly as
+// QUADPACK does with it, ends with a NaN result, and the integral is
+// counted as failed, where the reference process would have stopped.
 struct FluxFormIntegrand {
   const double4* __restrict__ ff;
   double b, w2g2;
@@ -164,7 +171,8 @@ struct FluxFormIntegrand {
     const double j1 = bessel_j1(b * q * (1.0 / kHc));
     const double d = tc - knot(kQ2min, kDQ2, i);
     const double f = c.x + d * (c.y + d * (c.z + d * c.w));
-    return q * q * f * rcp_nr(t) * j1;  // t >= (k / g1)^2 > 0
+    const double v = q * q * f * rcp_nr(t) * j1;  // t >= (k / g1)^2 > 0
+    return tc > kQ2Last ? __builtin_nan("") : v;
   }
 };
 
@@ -194,6 +202,7 @@ struct FluxTask {
   const double4* ff;  // form-factor spline of the task's nucleus
   double g1, factor;
   int A;
+  int slot;  // job of a batched task (0 otherwise)
 };
 
 // Task sources of the QAGS kernel: get(t) describes task t.
@@ -214,7 +223,10 @@ struct GridTask {  // flux-row nodes of one context: id = row * 128 + j, (b, k) 
     ft->g1 = fp.g1;
     ft->factor = fp.factor;
     ft->A = fp.A;
+    ft->slot = 0;
   }
+  // An integral that ended with ier != 0 (counted in the launch's stats).
+  __device__ void failed(const FluxTask&) const {}
 };
 
 struct PointTask {  // arbitrary (b, k) points of one context
@@ -233,7 +245,9 @@ struct PointTask {  // arbitrary (b, k) points of one context
     ft->g1 = fp.g1;
     ft->factor = fp.factor;
     ft->A = fp.A;
+    ft->slot = 0;
   }
+  __device__ void failed(const FluxTask&) const {}
 };
 
 // ---------------------------------------------------------------------------
@@ -297,9 +311,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(UPC_QAGS_MI
     if (st == kQagsDone) {
       const double Q = q.result / ft.A;
       flux[ft.out] = ft.factor * Q * Q / ft.k;
-      if (ier_out) ier_out[ft.out] = q.ier;
+      const int status = q.status();
+      if (ier_out) ier_out[ft.out] = status;
       my_evals += (unsigned long long)q.neval();
-      my_bad += q.ier != 0;
+      my_bad += status != 0;
+      if (status != 0) ts.failed(ft);
       task = -1;
     } else if (st == kQagsOverflow) {
       const unsigned long long slot = atomicAdd(&stats[2], 1ull);
@@ -347,7 +363,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(UPC_QAGS_MI
       for (int k = 0; k < np; ++k) {  // one copy of the integrand for both passes
         double qa, qb;
         q.interval(k, &qa, &qb);
+#ifdef UPC_QK21_REGS
+        const QK21Out r = qk21(f, qa, qb);
+#elif defined(UPC_QK21_ROLLED_REGS)
+        const QK21Out r = qk21_rolled(f, qa, qb);
+#else
         const QK21Out r = qk21_lds(f, qa, qb, fv, 64);
+#endif
         if (k == 0) o = r;
         else o2 = r;
       }
@@ -403,9 +425,13 @@ __global__ __launch_bounds__(64) void k_flux_qags_big(TS ts, const int* __restri
     }
     const double Q = q.result / ft.A;
     flux[ft.out] = ft.factor * Q * Q / ft.k;
-    if (ier_out) ier_out[ft.out] = st == kQagsDone ? q.ier : 99;
+    const int status = st == kQagsDone ? q.status() : 99;
+    if (ier_out) ier_out[ft.out] = status;
     atomicAdd(&stats[0], (unsigned long long)q.neval());
-    if (st != kQagsDone || q.ier != 0) atomicAdd(&stats[1], 1ull);
+    if (status != 0) {
+      atomicAdd(&stats[1], 1ull);
+      ts.failed(ft);
+    }
   }
 }
 

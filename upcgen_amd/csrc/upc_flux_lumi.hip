@@ -70,13 +70,19 @@ static int qags_persistent_blocks()
   (void)hipGetDevice(&dev);
   (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
   if (ncu <= 0) ncu = 256;
-  return ncu * 4;
+  return ncu * UPC_QAGS_MIN_WAVES;  // 256-lane workgroups: one wave per SIMD each
 }
 
-static int ensure_work(upc_ctx* c, size_t npts)
+// Work buffers for npts points and their flux rows (at most 2 npts).  QAGS
+// task ids are row * 128 + node in 32 bits and row offsets are int, so more
+// than 2^24 - 1 rows are refused here, before anything is allocated or run.
+static int ensure_work(upc_ctx* c, size_t npts, size_t rows)
 {
   LumiWork& w = c->work;
-  const size_t rows = 2 * npts;
+  if (rows * 128 >= (size_t(1) << 31))
+    return set_error("too many flux rows for 32-bit QAGS task ids (" + std::to_string(rows) +
+                     " rows, at most 16777215)"),
+           UPC_EINVAL;
   if (rows <= w.rows_cap && npts <= w.points_cap) return 0;
   free_work(c);
   UPC_HIP(hipMalloc(&w.d_row_k, sizeof(double) * rows));
@@ -114,7 +120,10 @@ static int grid_mirror_ny(const upc_ctx* c, bool grid)
   return grid && c->cfg.ymin == -c->cfg.ymax ? c->cfg.ny : 0;
 }
 
-static int flux_rows(int npts, int mirror_ny) { return mirror_ny > 0 ? npts + npts / mirror_ny : 2 * npts; }
+static long long flux_rows(int npts, int mirror_ny)
+{
+  return mirror_ny > 0 ? (long long)npts + npts / mirror_ny : 2LL * npts;
+}
 
 // Runs flux rows + QAGS for the points already in c->work.d_points.
 static int run_flux(upc_ctx* c, int npts, int mirror_ny, hipStream_t s)
@@ -163,7 +172,7 @@ int make_grid_points(upc_ctx* c, hipStream_t s)
 {
   const upc_config& g = c->cfg;
   const long npts = (long)g.nm * g.ny;
-  int rc = ensure_work(c, (size_t)npts);
+  int rc = ensure_work(c, (size_t)npts, (size_t)flux_rows((int)npts, grid_mirror_ny(c, true)));
   if (rc) return rc;
   hipLaunchKernelGGL(k_grid_points, dim3(nblk(npts, 256)), dim3(256), 0, s, g.nm, g.mmin, g.mmax, g.ny,
                      g.ymin, g.ymax, c->work.d_points);
@@ -184,13 +193,13 @@ __global__ void k_copy_points(const double* __restrict__ M, const double* __rest
 
 int prepare_flux(upc_ctx* c, const double* d_pts, int npts, FluxParams* fp, hipStream_t s)
 {
-  int rc = ensure_work(c, (size_t)npts);
+  const int mirror_ny = grid_mirror_ny(c, d_pts == nullptr);
+  int rc = ensure_work(c, (size_t)npts, (size_t)flux_rows(npts, mirror_ny));
   if (rc) return rc;
   if (d_pts) {
     hipLaunchKernelGGL(k_copy_points, dim3(nblk(npts, 256)), dim3(256), 0, s, d_pts, d_pts + npts, npts,
                        c->work.d_points);
   }
-  const int mirror_ny = grid_mirror_ny(c, d_pts == nullptr);
   rc = run_flux(c, npts, mirror_ny, s);
   if (rc) return rc;
   *fp = flux_params(c, mirror_ny);
@@ -361,26 +370,47 @@ int launch_flux_point(upc_ctx* c, const double* d_b, const double* d_k, int n, d
   return 0;
 }
 
+__global__ void k_zero_u64(unsigned long long* __restrict__ p, int n)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+// The per-point form-factor flux on the buffers in w: classify (point flux
+// beyond 2R, QAGS tasks inside), first QK21 passes, the persistent QAGS
+// kernel, then the 1000-interval kernel for the tasks that outgrew the fast
+// list.  Counters are cleared by a kernel, so the sequence may be captured.
+int enqueue_flux_form(upc_ctx* c, const double* d_b, const double* d_k, int n, double* d_out, int* d_ier,
+                      const FluxFormBufs& w, hipStream_t s)
+{
+  const FluxParams fp = flux_params(c);
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, w.counters, 8);
+  hipLaunchKernelGGL(k_flux_form_classify, dim3(nblk(n, 256)), dim3(256), 0, s, d_b, d_k, n, fp, d_out, d_ier,
+                     w.idx, w.counters);
+  PointTask ts{d_b, d_k, w.idx, fp, c->d_ff};
+  hipLaunchKernelGGL((k_flux_first<PointTask>), dim3(qags_persistent_blocks()), dim3(256), 0, s, ts,
+                     w.counters + 4, w.first);
+  hipLaunchKernelGGL((k_flux_qags<kQagsCap, PointTask>), dim3(qags_persistent_blocks()), dim3(256), 0, s, ts,
+                     w.counters + 4, w.counters + 5, d_out, d_ier, w.counters, w.overflow, c->qags_cap,
+                     (const double4*)w.first);
+  hipLaunchKernelGGL((k_flux_qags_big<PointTask>), dim3(w.nbig / 64), dim3(64), 0, s, ts, w.overflow, w.counters,
+                     d_out, d_ier, w.big, w.nbig);
+  UPC_HIP(hipGetLastError());
+  return 0;
+}
+
 int launch_flux_form(upc_ctx* c, const double* d_b, const double* d_k, int n, double* d_out, int* d_ier,
                      hipStream_t s)
 {
   if (n <= 0) return 0;
-  int rc = ensure_work(c, (size_t)(n + 1) / 2 + 1);
+  // the per-point QAGS uses the task arrays (rows x 120 ints) for its n indices
+  int rc = ensure_work(c, 1, (size_t)(n + kNB1 - 1) / kNB1);
   if (rc) return rc;
   LumiWork& w = c->work;
-  const FluxParams fp = flux_params(c);
-  UPC_HIP(hipMemsetAsync(w.d_counters, 0, sizeof(unsigned long long) * 8, s));
-  hipLaunchKernelGGL(k_flux_form_classify, dim3(nblk(n, 256)), dim3(256), 0, s, d_b, d_k, n, fp, d_out, d_ier,
-                     w.d_tasks, w.d_counters);
-  PointTask ts{d_b, d_k, w.d_tasks, fp, c->d_ff};
-  hipLaunchKernelGGL((k_flux_first<PointTask>), dim3(qags_persistent_blocks()), dim3(256), 0, s, ts,
-                     w.d_counters + 4, w.d_first);
-  hipLaunchKernelGGL((k_flux_qags<kQagsCap, PointTask>), dim3(qags_persistent_blocks()), dim3(256), 0, s, ts,
-                     w.d_counters + 4, w.d_counters + 5, d_out, d_ier, w.d_counters, w.d_overflow,
-                     c->qags_cap, (const double4*)w.d_first);
-  hipLaunchKernelGGL((k_flux_qags_big<PointTask>), dim3(kQagsBigLanes / 64), dim3(64), 0, s, ts, w.d_overflow,
-                     w.d_counters, d_out, d_ier, (QagsBigArrays*)w.d_big, kQagsBigLanes);
-  UPC_HIP(hipGetLastError());
+  const FluxFormBufs bufs{w.d_tasks, w.d_first, w.d_overflow, w.d_counters, (QagsBigArrays*)w.d_big,
+                          kQagsBigLanes};
+  rc = enqueue_flux_form(c, d_b, d_k, n, d_out, d_ier, bufs, s);
+  if (rc) return rc;
   return read_stats(c, s);
 }
 

@@ -53,6 +53,18 @@ struct LumiWork {
 
 constexpr int kQagsBigLanes = 256;
 
+struct QagsBigArrays;  // upc_qags.h
+
+// Buffers of one per-point form-factor flux launch (upc_flux_form).
+struct FluxFormBufs {
+  int* idx;                      // [n] compacted indices of the points that need QAGS
+  double4* first;                // [n] their first QK21 passes
+  int* overflow;                 // [n] tasks that outgrew the fast interval list
+  unsigned long long* counters;  // [8] evaluations, failed, overflow count, -, tasks, cursor
+  QagsBigArrays* big;            // [nbig] 1000-interval QAGS workspaces
+  int nbig;                      // a multiple of 64
+};
+
 // upc_events.hip
 int build_pt_tables(upc_ctx* c, double gtot, hipStream_t s);
 void free_events(upc_ctx* c);
@@ -166,6 +178,8 @@ int launch_flux_point(upc_ctx* c, const double* d_b, const double* d_k, int n, d
                       hipStream_t s);
 int launch_flux_form(upc_ctx* c, const double* d_b, const double* d_k, int n, double* d_out,
                      int* d_ier, hipStream_t s);
+int enqueue_flux_form(upc_ctx* c, const double* d_b, const double* d_k, int n, double* d_out, int* d_ier,
+                      const FluxFormBufs& w, hipStream_t s);
 void free_work(upc_ctx* c);
 int launch_bessel_j1(const double* d_x, int n, double* d_out, hipStream_t s);
 int launch_photon_flux(upc_ctx* c, const double* d_M, const double* d_Y, int n, double* d_out, hipStream_t s);

@@ -375,6 +375,17 @@ struct Qags {
   }
 
   __device__ int neval() const { return 42 * last - 21; }
+
+  // Status of a finished integral as the reference sees it: dqagse's ier,
+  // or 100 when an integrand value was NaN (the form-factor spline's
+  // GSL_EDOM) without an ier.  A NaN value makes area and errsum NaN for
+  // good, and the result NaN unless an earlier extrapolation is returned.
+  __device__ int status() const
+  {
+    if (ier != 0) return ier;
+    const bool nan = result != result || (last > 1 && (errsum != errsum || area != area));
+    return nan ? 100 : 0;
+  }
 };
 
 }  // namespace upc

@@ -41,7 +41,8 @@ def features(config: Config, M, Y, stream=None):
     n = M.numel()
     X = torch.empty((n, IN), dtype=torch.float32, device="cuda")
     L0 = torch.empty(n, dtype=torch.float64, device="cuda")
-    _check(load().upc_sur_features(C.byref(config), _dptr(M), _dptr(Y), n, _dptr(X), _dptr(L0), _stream(stream)),
+    _check(load().upc_sur_features(C.byref(config), _dptr(M, numel=n), _dptr(Y, numel=n), n,
+                                   _dptr(X, torch.float32), _dptr(L0), _stream(stream)),
            "upc_sur_features")
     return X, L0
 
@@ -120,20 +121,27 @@ class Surrogate:
         torch = _dev()
         B = X.shape[0]
         out = torch.empty(B, dtype=torch.float32, device="cuda")
-        _check(load().upc_sur_forward(self._s, net, _dptr(X), B, _dptr(out), _stream(stream)), "upc_sur_forward")
+        _check(load().upc_sur_forward(self._s, net, _dptr(X, torch.float32, B * IN, name="X"), B,
+                                      _dptr(out, torch.float32), _stream(stream)), "upc_sur_forward")
         return out
 
     def grad(self, net: int, X, T, stream=None):
         torch = _dev()
         g = torch.empty(self.nparams, dtype=torch.float32, device="cuda")
-        _check(load().upc_sur_grad(self._s, net, _dptr(X), _dptr(T), X.shape[0], _dptr(g), _stream(stream)),
+        B = X.shape[0]
+        _check(load().upc_sur_grad(self._s, net, _dptr(X, torch.float32, B * IN, name="X"),
+                                   _dptr(T, torch.float32, B, name="T"), B, _dptr(g, torch.float32),
+                                   _stream(stream)),
                "upc_sur_grad")
         return g
 
     def train(self, X, T, steps=4000, batch=4096, lr=2e-3, graph=True, stream=None):
         """Adam on both networks; returns the per-step loss curves [2][steps]."""
         loss = np.zeros((2, steps), dtype=np.float32)
-        _check(load().upc_sur_train(self._s, _dptr(X), _dptr(T), X.shape[0], steps, batch, lr, int(graph),
+        import torch
+        n = X.shape[0]
+        _check(load().upc_sur_train(self._s, _dptr(X, torch.float32, n * IN, name="X"),
+                                    _dptr(T, torch.float32, n, name="T"), n, steps, batch, lr, int(graph),
                                     _stream(stream), loss.ctypes.data_as(C.c_void_p)), "upc_sur_train")
         return loss
 
