@@ -248,6 +248,8 @@ typedef struct upc_batch_stats {
   long long points, rows;
   long long qags_calls, qags_evals, qags_bad;   /* of the last run */
   int qags_blocks_per_cu, lumi_waves;           /* launch shapes in use */
+  long long tasks;        /* form-factor flux integrals per run (counted at creation) */
+  long long work_bytes;   /* device memory of the batch's work buffers (the jobs' tables not included) */
 } upc_batch_stats;
 
 /* Builds one context (tables) per job and the batch work buffers. */

@@ -2,12 +2,28 @@
 // does not reach (calcPhotonFlux, calcNucCrossSectionY, getPhotonPt,
 // getPairMomentum) and prints their results as one JSON object for
 // tests/test_gpu_facade.py.  Built by __graft_entry__.build(); needs a GPU.
+#include <cmath>
+#include <cstdarg>
 #include <cstdio>
+#include <string>
 #include <vector>
 
 #include "UpcGenerator.h"
 
 namespace {
+// The object is built up in `out` and printed once at the end: the facade's
+// log lines go to the same stdout while the members run.
+std::string out;
+void emit(const char* fmt, ...)
+{
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  std::vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  out += buf;
+}
+
 // an elementary process with a known sigma(y) for calcNucCrossSectionY
 struct LinearY : UpcElemProcess {
   double calcCrossSectionY(double y) override { return 1.0 + 0.1 * y + 0.01 * y * y; }
@@ -32,19 +48,19 @@ int main()
   proc->mPart = 3.0969;
   cs.elemProcess = proc;
 
-  std::printf("{\"photon_flux\": [");
+  emit("{\"photon_flux\": [");
   const double ys[] = {-5.0, -2.0, 0.0, 1.5, 4.0};
-  for (int i = 0; i < 5; ++i) std::printf("%s[%.17g, %.17g]", i ? ", " : "", ys[i], cs.calcPhotonFlux(3.0969, ys[i]));
-  std::printf("], ");
+  for (int i = 0; i < 5; ++i) emit("%s[%.17g, %.17g]", i ? ", " : "", ys[i], cs.calcPhotonFlux(3.0969, ys[i]));
+  emit("], ");
 
   std::vector<std::vector<double>> csY, ratio;
   double tot = 0.;
   cs.calcNucCrossSectionY(csY, ratio, tot);
-  std::printf("\"cs_y\": [");
-  for (size_t i = 0; i < csY.size(); ++i) std::printf("%s%.17g", i ? ", " : "", csY[i][0]);
-  std::printf("], \"ratio_y\": [");
-  for (size_t i = 0; i < ratio.size(); ++i) std::printf("%s%.17g", i ? ", " : "", ratio[i][0]);
-  std::printf("], \"tot_y\": %.17g, ", tot);
+  emit("\"cs_y\": [");
+  for (size_t i = 0; i < csY.size(); ++i) emit("%s%.17g", i ? ", " : "", csY[i][0]);
+  emit("], \"ratio_y\": [");
+  for (size_t i = 0; i < ratio.size(); ++i) emit("%s%.17g", i ? ", " : "", ratio[i][0]);
+  emit("], \"tot_y\": %.17g, ", tot);
 
   // photon pt at E = 2 GeV: mean and second moment of 4000 draws
   double s1 = 0., s2 = 0.;
@@ -54,7 +70,7 @@ int main()
     s1 += pt;
     s2 += pt * pt;
   }
-  std::printf("\"pt_mean\": %.17g, \"pt_rms\": %.17g, \"gtot\": %.17g, ", s1 / n, std::sqrt(s2 / n), cs.gtot);
+  emit("\"pt_mean\": %.17g, \"pt_rms\": %.17g, \"gtot\": %.17g, ", s1 / n, std::sqrt(s2 / n), cs.gtot);
 
   // pair momenta: invariant mass, rapidity and transverse momentum
   double dm = 0., dy = 0., ptsum = 0.;
@@ -69,7 +85,9 @@ int main()
   cs.useNonzeroGamPt = false;
   UpcLorentzVector p0;
   cs.getPairMomentum(10.0, 1.2, p0);
-  std::printf("\"pair_max_dm\": %.17g, \"pair_max_dy\": %.17g, \"pair_mean_pt\": %.17g, \"pair_pt_zero\": %.17g}\n",
+  emit("\"pair_max_dm\": %.17g, \"pair_max_dy\": %.17g, \"pair_mean_pt\": %.17g, \"pair_pt_zero\": %.17g}\n",
               dm, dy, ptsum / 500, p0.Pt());
+  std::fflush(stdout);
+  std::fputs(out.c_str(), stdout);
   return 0;
 }

@@ -62,18 +62,23 @@ def test_rejects_short_sampler_tables(gpu, ctx):
         ctx.sample_events(sum_ym, n=10)
 
 
-def test_rejects_grids_beyond_32bit_task_ids(gpu):
+def test_grid_beyond_2_pow_24_flux_rows(gpu, oracle_ctx):
     """An asymmetric Y range gives every point its own k2 row: 4096 x 2049
-    points need 16,785,408 flux rows, one past what row * 128 + node fits in
-    an int.  The library refuses the grid before allocating or launching."""
-    c = gpu.Context(flux_point=0, breakup_mode=2, nm=4096, ny=2049, ymin=-5.0, ymax=6.0)
+    points have 16,785,408 flux rows, past 2^24 where row * 128 + node no
+    longer fits 31 bits.  Task records, row offsets and flux slots are 64-bit;
+    with the point-charge flux (no QAGS tasks, 16 GB of flux rows) the grid's
+    last rows must still match the oracle."""
+    import torch
+    nm, ny, ymin, ymax = 4096, 2049, -5.0, 6.0
+    c = gpu.Context(flux_point=1, breakup_mode=1, nm=nm, ny=ny, ymin=ymin, ymax=ymax)
     try:
-        with pytest.raises(gpu.UpcError, match="32-bit QAGS task ids"):
-            c.lumi_grid()
-        import torch
-        n = 16_777_216 // 2 + 1  # points with two rows each: 2^24 + 2 rows
-        M = torch.full((n,), 10.0, dtype=torch.float64, device="cuda")
-        with pytest.raises(gpu.UpcError, match="32-bit QAGS task ids"):
-            c.lumi_points(M, M)
+        L = c.lumi_grid()
+        torch.cuda.synchronize()
+        o = oracle_ctx(flux_point=1, breakup_mode=1, nm=3, ny=3)
+        dm, dy = (50 - 3.56) / nm, (ymax - ymin) / ny
+        for im, iy in ((nm - 1, ny - 1), (nm - 1, 0), (nm // 2, ny // 3), (0, ny - 1)):
+            ref = o.lumi(3.56 + dm * im, ymin + dy * iy) * dm * dy
+            got = float(L[im, iy])
+            assert abs(got / ref - 1) < 1e-9, (im, iy, got, ref)
     finally:
         c.close()

@@ -82,7 +82,8 @@ class Tune(C.Structure):
 class BatchStats(C.Structure):
     _fields_ = [("njobs", C.c_int), ("points", C.c_longlong), ("rows", C.c_longlong),
                 ("qags_calls", C.c_longlong), ("qags_evals", C.c_longlong), ("qags_bad", C.c_longlong),
-                ("qags_blocks_per_cu", C.c_int), ("lumi_waves", C.c_int)]
+                ("qags_blocks_per_cu", C.c_int), ("lumi_waves", C.c_int), ("tasks", C.c_longlong),
+                ("work_bytes", C.c_longlong)]
 
 
 # Every symbol declared in include/upc_lumi.h with its ctypes signature.

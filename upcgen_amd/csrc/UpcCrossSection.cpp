@@ -237,7 +237,10 @@ upc_config UpcCrossSection::config() const
   c.nz = nz;
   c.zmin = zmin;
   c.zmax = zmax;
-  c.proc_id = elemProcess ? elemProcess->partPDG : 13;
+  // the context's own elementary tables cover the dileptons; another process
+  // (a caller's vector meson for calcNucCrossSectionY) leaves them at the muon's
+  const int pdg = elemProcess ? elemProcess->partPDG : 13;
+  c.proc_id = (pdg == 11 || pdg == 13 || pdg == 15) ? pdg : 13;
   auto* dl = dynamic_cast<UpcTwoPhotonDilep*>(elemProcess);
   c.lep_a = dl ? dl->aLep : 0.0;
   return c;

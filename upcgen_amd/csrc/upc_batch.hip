@@ -16,6 +16,7 @@
 #include "upc_flux.h"
 
 #include <algorithm>
+#include <climits>
 #include <vector>
 
 struct upc_batch;
@@ -78,34 +79,31 @@ __global__ void k_batch_rows(const BatchJob* __restrict__ jobs, const int* __res
 __global__ void k_batch_point_rows(const BatchJob* __restrict__ jobs, const int* __restrict__ row0, int njobs,
                                    long total_rows, const double* __restrict__ row_k,
                                    const double* __restrict__ row_logd, const int* __restrict__ row_cnt,
-                                   const int* __restrict__ row_off, double* __restrict__ flux,
-                                   int* __restrict__ tasks, double2* __restrict__ task_bk)
+                                   const long long* __restrict__ row_off, double* __restrict__ flux,
+                                   TaskRec* __restrict__ recs)
 {
   const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= total_rows * kNB1) return;
   const long r = gid / kNB1;
   const BatchJob& J = jobs[find_job(row0, njobs, r)];
-  point_row_node((int)r, (int)(gid % kNB1), J.v.fp, row_k, row_logd, row_cnt, row_off, flux, tasks, task_bk);
+  point_row_node(r, (int)(gid % kNB1), J.v.fp, row_k, row_logd, row_cnt, row_off, flux, recs);
 }
 
-// QAGS task source over all jobs: task id = global row * 128 + node.
+// QAGS task source over all jobs: task records of the concatenated flux rows.
 struct BatchTask {
-  const int* __restrict__ tasks;
-  const double2* __restrict__ bk;
+  const TaskRec* __restrict__ recs;
   const BatchJob* __restrict__ jobs;
   const int* __restrict__ row0;
   int njobs;
   unsigned long long* __restrict__ job_failed;  // [njobs] integrals with ier != 0
-  __device__ void get(int t, FluxTask* ft) const
+  __device__ void get(long long t, FluxTask* ft) const
   {
-    const int id = tasks[t];
-    const double2 v = bk[t];
-    const int r = id >> 7, j = id & 127;
-    const int jb = find_job(row0, njobs, r);
+    const TaskRec rec = recs[t];
+    const int jb = find_job(row0, njobs, rec.out / kNB1);
     const BatchJob& J = jobs[jb];
-    ft->b = v.x;
-    ft->k = v.y;
-    ft->out = (long)r * kNB1 + j;
+    ft->b = rec.b;
+    ft->k = rec.k;
+    ft->out = rec.out;
     ft->ff = J.ff;
     ft->g1 = J.v.fp.g1;
     ft->factor = J.v.fp.factor;
@@ -148,17 +146,16 @@ struct upc_batch {
   double* d_row_k = nullptr;
   double* d_row_logd = nullptr;
   int* d_row_cnt = nullptr;
-  int* d_row_off = nullptr;
+  long long* d_row_off = nullptr;
   long long* d_part = nullptr;
   double* d_flux = nullptr;
-  int* d_tasks = nullptr;
-  double2* d_task_bk = nullptr;
+  long long ntasks = 0;          // QAGS tasks of every job (counted at creation)
+  long long work_bytes = 0;      // device memory of the buffers below
+  TaskRec* d_tasks = nullptr;
   double4* d_first = nullptr;  // first QK21 pass of each task (k_flux_first)
-  int* d_overflow = nullptr;
+  long long* d_overflow = nullptr;
   unsigned long long* d_counters = nullptr;  // [0] evals [1] bad [2] overflow n [4] tasks [5] cursor
   unsigned long long* d_job_failed = nullptr;  // [njobs] QAGS integrals with ier != 0 per job
-  QagsBigArrays* d_big = nullptr;
-  int nbig = 256;
   int qags_choice = 2, lumi_choice = 1;  // indices into the candidate lists
   hipStream_t cap_stream = nullptr;
   hipGraph_t graph = nullptr;
@@ -185,7 +182,7 @@ void free_batch(upc_batch* b)
   if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
   void* ptrs[] = {b->d_jobs, b->d_pt0, b->d_row0, b->d_blk0[0], b->d_blk0[1], b->d_blk0[2], b->d_pts,
                   b->d_row_k, b->d_row_logd, b->d_row_cnt, b->d_row_off, b->d_part, b->d_flux, b->d_tasks,
-                  b->d_task_bk, b->d_first, b->d_overflow, b->d_counters, b->d_job_failed, b->d_big};
+                  b->d_first, b->d_overflow, b->d_counters, b->d_job_failed};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (upc_ctx* c : b->ctx) upc_ctx_destroy(c);
@@ -207,17 +204,16 @@ int enqueue(upc_batch* b, hipStream_t s)
   hipLaunchKernelGGL(k_scan_apply<>, dim3(b->nparts), dim3(kScanBlock), 0, s, b->d_row_cnt, b->nrows, b->d_part,
                      b->d_row_off);
   hipLaunchKernelGGL(k_batch_point_rows, dim3(nblk(b->nrows * kNB1, 256)), dim3(256), 0, s, b->d_jobs, b->d_row0,
-                     nj, b->nrows, b->d_row_k, b->d_row_logd, b->d_row_cnt, b->d_row_off, b->d_flux, b->d_tasks,
-                     b->d_task_bk);
-  BatchTask ts{b->d_tasks, b->d_task_bk, b->d_jobs, b->d_row0, nj, b->d_job_failed};
+                     nj, b->nrows, b->d_row_k, b->d_row_logd, b->d_row_cnt, b->d_row_off, b->d_flux, b->d_tasks);
+  BatchTask ts{b->d_tasks, b->d_jobs, b->d_row0, nj, b->d_job_failed};
   const int qblocks = b->ncu * kQagsBlockChoices[b->qags_choice];
   hipLaunchKernelGGL((k_flux_first<BatchTask>), dim3(b->ncu * kQagsBlockChoices[2]), dim3(256), 0, s, ts,
                      b->d_counters + 4, b->d_first);
   hipLaunchKernelGGL((k_flux_qags<kQagsCap, BatchTask>), dim3(qblocks), dim3(256), 0, s, ts, b->d_counters + 4,
                      b->d_counters + 5, b->d_flux, (int*)nullptr, b->d_counters, b->d_overflow,
                      kQagsCap, (const double4*)b->d_first);
-  hipLaunchKernelGGL((k_flux_qags_big<BatchTask>), dim3(b->nbig / 64), dim3(64), 0, s, ts, b->d_overflow,
-                     b->d_counters, b->d_flux, (int*)nullptr, b->d_big, b->nbig);
+  hipLaunchKernelGGL((k_flux_qags_wave<BatchTask>), dim3(kQagsWaveBlocks), dim3(64), 0, s, ts, b->d_overflow,
+                     b->d_counters, b->d_flux, (int*)nullptr);
   const int li = b->lumi_choice, w = kLumiWaveChoices[li];
   BatchLumi jr{b->d_jobs, b->d_blk0[li], nj};
   if (w == 2) hipLaunchKernelGGL((k_lumi_fp64<2, BatchLumi>), dim3(b->lumi_blocks[li]), dim3(128), 0, s, jr);
@@ -265,15 +261,19 @@ int upc_batch_create(const upc_config* cfgs, int njobs, void* stream, upc_batch*
     np += n;
     nr += mirror ? n + g.nm : 2 * n;
   }
-  if (nr * 128 >= (1L << 31)) {
+  if (np > INT_MAX || nr > INT_MAX) {
     free_batch(b);
-    return set_error("batch too large for 32-bit task ids"), UPC_EINVAL;
+    return set_error("batch too large: more than 2^31 - 1 points or flux rows"), UPC_EINVAL;
   }
   b->npts = np;
   b->nrows = nr;
   b->nparts = nblk(nr, kScanBlock);
   for (int i = 0; i < 3; ++i) b->lumi_blocks[i] = (int)blocks[i];
-#define BA(ptr, n) UPC_HIP(hipMalloc(&(ptr), sizeof(*(ptr)) * (size_t)(n)))
+#define BA(ptr, n)                                                  \
+  do {                                                              \
+    UPC_HIP(hipMalloc(&(ptr), sizeof(*(ptr)) * (size_t)(n)));       \
+    b->work_bytes += (long long)(sizeof(*(ptr)) * (size_t)(n));     \
+  } while (0)
   auto alloc = [&]() -> int {
     BA(b->d_jobs, njobs);
     BA(b->d_pt0, njobs);
@@ -286,13 +286,8 @@ int upc_batch_create(const upc_config* cfgs, int njobs, void* stream, upc_batch*
     BA(b->d_row_off, nr);
     BA(b->d_part, b->nparts);
     BA(b->d_flux, nr * kNB1);
-    BA(b->d_tasks, nr * kNB1);
-    BA(b->d_task_bk, nr * kNB1);
-    BA(b->d_first, nr * kNB1);
-    BA(b->d_overflow, nr * kNB1);
     BA(b->d_counters, 8);
     BA(b->d_job_failed, njobs);
-    BA(b->d_big, b->nbig);
     return 0;
   };
 #undef BA
@@ -337,6 +332,34 @@ int upc_batch_create(const upc_config* cfgs, int njobs, void* stream, upc_batch*
     return 0;
   };
   if (int rc = upload()) {
+    free_batch(b);
+    return rc;
+  }
+  // The QAGS task buffers (records, first passes, overflow list) are sized by
+  // the batch's task count, which the row kernels and the scan give once here.
+  auto size_tasks = [&]() -> int {
+    const int nj = (int)b->jobs.size();
+    hipLaunchKernelGGL(k_batch_zero, dim3(1), dim3(64), 0, s, b->d_counters, 8);
+    hipLaunchKernelGGL(k_batch_points, dim3(nblk(b->npts, 256)), dim3(256), 0, s, b->d_jobs, b->d_pt0, nj,
+                       b->npts, b->d_pts);
+    hipLaunchKernelGGL(k_batch_rows, dim3(nblk(b->nrows, 256)), dim3(256), 0, s, b->d_jobs, b->d_row0, nj,
+                       b->nrows, b->d_row_k, b->d_row_logd, b->d_row_cnt);
+    hipLaunchKernelGGL(k_scan_part<>, dim3(b->nparts), dim3(kScanBlock), 0, s, b->d_row_cnt, b->nrows,
+                       b->d_part);
+    hipLaunchKernelGGL(k_scan_top<>, dim3(1), dim3(kScanBlock), 0, s, b->d_part, b->nparts, b->d_counters + 4);
+    UPC_HIP(hipGetLastError());
+    unsigned long long total = 0;
+    UPC_HIP(hipMemcpyAsync(&total, b->d_counters + 4, sizeof(total), hipMemcpyDeviceToHost, s));
+    UPC_HIP(hipStreamSynchronize(s));
+    b->ntasks = (long long)total;
+    const size_t n = total > 0 ? (size_t)total : 1;
+    UPC_HIP(hipMalloc(&b->d_tasks, sizeof(TaskRec) * n));
+    UPC_HIP(hipMalloc(&b->d_first, sizeof(double4) * n));
+    UPC_HIP(hipMalloc(&b->d_overflow, sizeof(long long) * n));
+    b->work_bytes += (long long)((sizeof(TaskRec) + sizeof(double4) + sizeof(long long)) * n);
+    return 0;
+  };
+  if (int rc = size_tasks()) {
     free_batch(b);
     return rc;
   }
@@ -411,7 +434,7 @@ int upc_batch_autotune(upc_batch* b, void* stream, upc_tune* h_out)
   UPC_HIP(hipEventCreate(&e0));
   UPC_HIP(hipEventCreate(&e1));
   const int nj = (int)b->jobs.size();
-  BatchTask ts{b->d_tasks, b->d_task_bk, b->d_jobs, b->d_row0, nj, b->d_job_failed};
+  BatchTask ts{b->d_tasks, b->d_jobs, b->d_row0, nj, b->d_job_failed};
   for (int i = 0; i < 3; ++i) {
     // QAGS alone (the first passes of the warm run stay valid: they depend on
     // the task list only): clear the counters except the task total, then time
@@ -430,8 +453,8 @@ int upc_batch_autotune(upc_batch* b, void* stream, upc_tune* h_out)
     t.qags_ms[i] = ms;
   }
   // restore the overflow results (the timed passes re-ran every task)
-  hipLaunchKernelGGL((k_flux_qags_big<BatchTask>), dim3(b->nbig / 64), dim3(64), 0, s, ts, b->d_overflow,
-                     b->d_counters, b->d_flux, (int*)nullptr, b->d_big, b->nbig);
+  hipLaunchKernelGGL((k_flux_qags_wave<BatchTask>), dim3(kQagsWaveBlocks), dim3(64), 0, s, ts, b->d_overflow,
+                     b->d_counters, b->d_flux, (int*)nullptr);
   for (int i = 0; i < 3; ++i) {
     BatchLumi jr{b->d_jobs, b->d_blk0[i], nj};
     UPC_HIP(hipEventRecord(e0, s));
@@ -494,6 +517,8 @@ int upc_batch_info(upc_batch* b, void* stream, upc_batch_stats* h_out)
   h_out->qags_bad = (long long)h[1];
   h_out->qags_blocks_per_cu = kQagsBlockChoices[b->qags_choice];
   h_out->lumi_waves = kLumiWaveChoices[b->lumi_choice];
+  h_out->tasks = b->ntasks;
+  h_out->work_bytes = b->work_bytes;
   return 0;
 }
 

@@ -448,12 +448,26 @@ __device__ __forceinline__ QK21Out qk21(const F& f, double a, double b)
   return o;
 }
 
-// dqk21 as a rolled loop over the 21 nodes, for kernels where one integrand
-// evaluation is large: the same operations in the same order as qk21()
-// above, with the 20 off-centre values parked in LDS (fv[s * stride], s =
-// 0..19) instead of registers.
-template <class F>
-__device__ __forceinline__ QK21Out qk21_lds(const F& f, double a, double b, double* fv, int stride)
+// The 21 nodes of a QK21 pass in the order the passes below visit them:
+// e = 0 the centre, e = 1..10 the Gauss abscissae xgk(2j) in (left, right)
+// pairs, e = 11..20 the Kronrod-only abscissae xgk(2j - 1) in pairs.
+__device__ __forceinline__ double qk21_abscissa(int e, double centr, double hlgth)
+{
+  const bool gauss = e <= 10;
+  const int p = gauss ? (e - 1) >> 1 : (e - 11) >> 1;
+  const int side = gauss ? (e - 1) & 1 : (e - 11) & 1;
+  const int xi = e == 0 ? 10 : (gauss ? 2 * p + 1 : 2 * p);
+  const double absc = hlgth * kXgk[xi];
+  return e == 0 ? centr : (side ? centr + absc : centr - absc);
+}
+
+// dqk21 as a rolled loop over the 21 nodes: the same operations in the same
+// order as qk21() above, with the 20 off-centre values parked at
+// fv[s * stride] (s = 0..19) until the mean deviation needs them.  fx(e, x)
+// gives the integrand at node e, abscissa x (evaluated there, or read from
+// where other lanes evaluated it).
+template <class FX>
+__device__ __forceinline__ QK21Out qk21_sum(const FX& fx, double a, double b, double* fv, int stride)
 {
   constexpr double epmach = DBL_EPSILON, uflow = DBL_MIN;
   const double centr = 0.5 * (a + b);
@@ -462,22 +476,19 @@ __device__ __forceinline__ QK21Out qk21_lds(const F& f, double a, double b, doub
   double resg = 0.0, resk = 0.0, rabs = 0.0, fc = 0.0, f1 = 0.0;
 #pragma unroll 1
   for (int e = 0; e < 21; ++e) {
-    // e = 0: centre; 1..10: Gauss pairs (xgk 1,3,5,7,9); 11..20: Kronrod pairs (0,2,...,8)
     const bool gauss = e <= 10;
     const int p = gauss ? (e - 1) >> 1 : (e - 11) >> 1;
     const int side = gauss ? (e - 1) & 1 : (e - 11) & 1;
     const int xi = e == 0 ? 10 : (gauss ? 2 * p + 1 : 2 * p);
-    const double absc = hlgth * kXgk[xi];
-    const double x = e == 0 ? centr : (side ? centr + absc : centr - absc);
-    const double fx = f(x);
+    const double fx_e = fx(e, qk21_abscissa(e, centr, hlgth));
     if (e == 0) {
-      fc = fx;
+      fc = fx_e;
       resk = kWgk[10] * fc;
       rabs = fabs(resk);
     } else if (side == 0) {
-      f1 = fx;
+      f1 = fx_e;
     } else {
-      const double f2 = fx;
+      const double f2 = fx_e;
       fv[(2 * xi) * stride] = f1;
       fv[(2 * xi + 1) * stride] = f2;
       const double fsum = f1 + f2;
@@ -511,64 +522,11 @@ __device__ __forceinline__ QK21Out qk21_lds(const F& f, double a, double b, doub
   return o;
 }
 
-// dqk21 as the rolled loop of qk21_lds with the 20 off-centre values in a
-// local array indexed by the (wave-uniform) node counter.
+// dqk21 evaluating the integrand f at each node as it goes.
 template <class F>
-__device__ __forceinline__ QK21Out qk21_rolled(const F& f, double a, double b)
+__device__ __forceinline__ QK21Out qk21_lds(const F& f, double a, double b, double* fv, int stride)
 {
-  constexpr double epmach = DBL_EPSILON, uflow = DBL_MIN;
-  double fv[20];
-  const double centr = 0.5 * (a + b);
-  const double hlgth = 0.5 * (b - a);
-  const double dhlgth = fabs(hlgth);
-  double resg = 0.0, resk = 0.0, rabs = 0.0, fc = 0.0, f1 = 0.0;
-#pragma unroll 1
-  for (int e = 0; e < 21; ++e) {
-    const bool gauss = e <= 10;
-    const int p = gauss ? (e - 1) >> 1 : (e - 11) >> 1;
-    const int side = gauss ? (e - 1) & 1 : (e - 11) & 1;
-    const int xi = e == 0 ? 10 : (gauss ? 2 * p + 1 : 2 * p);
-    const double absc = hlgth * kXgk[xi];
-    const double x = e == 0 ? centr : (side ? centr + absc : centr - absc);
-    const double fx = f(x);
-    if (e == 0) {
-      fc = fx;
-      resk = kWgk[10] * fc;
-      rabs = fabs(resk);
-    } else if (side == 0) {
-      f1 = fx;
-    } else {
-      const double f2 = fx;
-      fv[2 * xi] = f1;
-      fv[2 * xi + 1] = f2;
-      const double fsum = f1 + f2;
-      if (gauss) resg = resg + kWg[p] * fsum;
-      resk = resk + kWgk[xi] * fsum;
-      rabs = rabs + kWgk[xi] * (fabs(f1) + fabs(f2));
-    }
-  }
-  const double reskh = resk * 0.5;
-  double rasc = kWgk[10] * fabs(fc - reskh);
-#pragma unroll
-  for (int j = 0; j < 10; ++j) rasc = rasc + kWgk[j] * (fabs(fv[2 * j] - reskh) + fabs(fv[2 * j + 1] - reskh));
-  QK21Out o;
-  o.result = resk * hlgth;
-  rabs = rabs * dhlgth;
-  rasc = rasc * dhlgth;
-  double err = fabs((resk - resg) * hlgth);
-  if (rasc != 0.0 && err != 0.0) {
-    const double t15 = 200.0 * err / rasc;
-    const double sc = t15 * sqrt(t15);
-    err = rasc * (sc < 1.0 ? sc : 1.0);
-  }
-  if (rabs > uflow / (50.0 * epmach)) {
-    const double m = (epmach * 50.0) * rabs;
-    if (m > err) err = m;
-  }
-  o.abserr = err;
-  o.resabs = rabs;
-  o.resasc = rasc;
-  return o;
+  return qk21_sum([&](int, double x) { return f(x); }, a, b, fv, stride);
 }
 
 // dqelg (Wynn epsilon) on a 1-based table of length >= 55.

@@ -176,22 +176,31 @@ struct FluxFormIntegrand {
   }
 };
 
-// Node j of flux row r: the point-charge flux (b > 2R or FLUX_POINT 1), or
-// a QAGS task id r * 128 + j at the row's task offset.
-__device__ __forceinline__ void point_row_node(int r, int j, const FluxParams& fp, const double* __restrict__ row_k,
-                                               const double* __restrict__ row_logd,
-                                               const int* __restrict__ row_cnt, const int* __restrict__ row_off,
-                                               double* __restrict__ flux, int* __restrict__ tasks,
-                                               double2* __restrict__ task_bk)
+// One form-factor QAGS task of a flux row: its node (b, k) and the flux slot
+// (row * 120 + node) the integral fills.
+struct TaskRec {
+  double b, k;
+  long long out;
+};
+
+// Node j of flux row r: the point-charge flux (b > 2R or FLUX_POINT 1), or a
+// QAGS task record at the row's task offset.
+__device__ __forceinline__ void point_row_node(long long r, int j, const FluxParams& fp,
+                                               const double* __restrict__ row_k, const double* __restrict__ row_logd,
+                                               const int* __restrict__ row_cnt,
+                                               const long long* __restrict__ row_off, double* __restrict__ flux,
+                                               TaskRec* __restrict__ recs)
 {
   double bl, bh;
   const double b = row_node(fp.bmin, row_logd[r], j, &bl, &bh);
   if (j < row_cnt[r]) {
-    tasks[row_off[r] + j] = r * 128 + j;
-    task_bk[row_off[r] + j] = make_double2(b, row_k[r]);
+    TaskRec* t = recs + row_off[r] + j;
+    t->b = b;
+    t->k = row_k[r];
+    t->out = r * kNB1 + j;
     return;
   }
-  flux[(long)r * kNB1 + j] = flux_point(b, row_k[r], fp.g1, fp.factor);
+  flux[r * kNB1 + j] = flux_point(b, row_k[r], fp.g1, fp.factor);
 }
 
 // One form-factor flux integral (fluxForm, src/UpcCrossSection.cpp:194-218):
@@ -206,19 +215,16 @@ struct FluxTask {
 };
 
 // Task sources of the QAGS kernel: get(t) describes task t.
-struct GridTask {  // flux-row nodes of one context: id = row * 128 + j, (b, k) beside it
-  const int* __restrict__ tasks;
-  const double2* __restrict__ bk;
+struct GridTask {  // flux-row nodes of one context
+  const TaskRec* __restrict__ recs;
   FluxParams fp;
   const double4* ff;
-  __device__ void get(int t, FluxTask* ft) const
+  __device__ void get(long long t, FluxTask* ft) const
   {
-    const int id = tasks[t];
-    const double2 v = bk[t];
-    const int r = id >> 7, j = id & 127;
-    ft->b = v.x;
-    ft->k = v.y;
-    ft->out = (long)r * kNB1 + j;
+    const TaskRec rec = recs[t];
+    ft->b = rec.b;
+    ft->k = rec.k;
+    ft->out = rec.out;
     ft->ff = ff;
     ft->g1 = fp.g1;
     ft->factor = fp.factor;
@@ -235,7 +241,7 @@ struct PointTask {  // arbitrary (b, k) points of one context
   const int* __restrict__ idx;  // compacted indices of points needing QAGS
   FluxParams fp;
   const double4* ff;
-  __device__ void get(int t, FluxTask* ft) const
+  __device__ void get(long long t, FluxTask* ft) const
   {
     const int i = idx[t];
     ft->b = b[i];
@@ -276,7 +282,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(UPC_QAGS_MI
     const long t = t0 + (threadIdx.x & ~63) + (threadIdx.x & 63);
     const bool live = t < ntasks;
     FluxTask ft{};
-    ts.get((int)(live ? t : ntasks - 1), &ft);
+    ts.get(live ? t : ntasks - 1, &ft);
     f.ff = ft.ff;
     f.b = ft.b;
     f.w2g2 = ft.k * ft.k / ft.g1 / ft.g1;
@@ -293,7 +299,7 @@ template <int CAP, class TS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(UPC_QAGS_MIN_WAVES, 8))) void k_flux_qags(
     TS ts, const unsigned long long* __restrict__ ntasks_p, unsigned long long* __restrict__ cursor,
     double* __restrict__ flux, int* __restrict__ ier_out, unsigned long long* __restrict__ stats,
-    int* __restrict__ overflow, int fast_cap, const double4* __restrict__ first)
+    long long* __restrict__ overflow, int fast_cap, const double4* __restrict__ first)
 {
   __shared__ double fvbuf[20 * 256];
   typename QagsSmallStore<CAP>::Arrays lists;
@@ -319,7 +325,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(UPC_QAGS_MI
       task = -1;
     } else if (st == kQagsOverflow) {
       const unsigned long long slot = atomicAdd(&stats[2], 1ull);
-      overflow[slot] = (int)task;
+      overflow[slot] = task;
       task = -1;
     }
   };
@@ -336,7 +342,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(UPC_QAGS_MI
         const long t = (long)(base + rank);
         if (t < ntasks) {
           task = t;
-          ts.get((int)t, &ft);
+          ts.get(t, &ft);
           f.ff = ft.ff;
           f.b = ft.b;
           f.w2g2 = ft.k * ft.k / ft.g1 / ft.g1;
@@ -363,13 +369,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(UPC_QAGS_MI
       for (int k = 0; k < np; ++k) {  // one copy of the integrand for both passes
         double qa, qb;
         q.interval(k, &qa, &qb);
-#ifdef UPC_QK21_REGS
-        const QK21Out r = qk21(f, qa, qb);
-#elif defined(UPC_QK21_ROLLED_REGS)
-        const QK21Out r = qk21_rolled(f, qa, qb);
-#else
         const QK21Out r = qk21_lds(f, qa, qb, fv, 64);
-#endif
         if (k == 0) o = r;
         else o2 = r;
       }
@@ -386,52 +386,75 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(UPC_QAGS_MI
   }
 }
 
-// Tasks that outgrew the fast interval list: rerun with the reference's
-// 1000-interval workspace, one per lane, workspace in global memory.
+// Tasks that outgrew the fast interval list (integrals that bisect more than
+// the fast list holds: in practice the ones whose integrand turned NaN or whose
+// roundoff tests fire, which run on to dozens or hundreds of intervals) are
+// rerun from the start here, one wavefront per integral, with the reference's
+// 1000-interval workspace in LDS.  Lanes 0..41 evaluate the 42 nodes of a
+// step (both halves) at once; lane 0 sums each pass in QK21's order (the same
+// qk21_sum as the fast kernel) and runs the dqagse state machine, so the
+// serial part of a step is LDS work and one sum per pass.
+constexpr int kQagsWaveBlocks = 1024;  // workgroups of the overflow kernel (36.5 KB LDS each)
+
 template <class TS>
-__global__ __launch_bounds__(64) void k_flux_qags_big(TS ts, const int* __restrict__ overflow,
-                                                      unsigned long long* __restrict__ stats,
-                                                      double* __restrict__ flux, int* __restrict__ ier_out,
-                                                      QagsBigArrays* __restrict__ ws, int nws)
+__global__ __launch_bounds__(64) void k_flux_qags_wave(TS ts, const long long* __restrict__ overflow,
+                                                       unsigned long long* __restrict__ stats,
+                                                       double* __restrict__ flux, int* __restrict__ ier_out)
 {
+  __shared__ QagsBigArrays ws;
+  __shared__ double vals[2][21];
+  __shared__ double fvs[20];
+  __shared__ double iv[2][2];
+  __shared__ int ctl[2];  // [0] passes of the step, [1] state after it
   const long n = (long)stats[2];
-  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= nws) return;
-  __shared__ double fvbuf[20 * 64];
-  double* fv = fvbuf + threadIdx.x;
-  Qags<QagsBigStore> q;
-  q.s.a = ws + gid;
-  FluxFormIntegrand f;
-  for (long i = gid; i < n; i += nws) {
+  const int lane = threadIdx.x;
+  for (long i = blockIdx.x; i < n; i += gridDim.x) {
     FluxTask ft;
     ts.get(overflow[i], &ft);
+    FluxFormIntegrand f;
     f.ff = ft.ff;
     f.b = ft.b;
     f.w2g2 = ft.k * ft.k / ft.g1 / ft.g1;
-    q.start();
+    Qags<QagsBigStore> q;
+    q.s.a = &ws;
+    if (lane == 0) q.start();
     int st = kQagsRunning;
-    while (st == kQagsRunning) {
-      QK21Out o, o2{};
-      const int np = q.passes();
-#pragma unroll 1
-      for (int k = 0; k < np; ++k) {
-        double qa, qb;
-        q.interval(k, &qa, &qb);
-        const QK21Out r = qk21_lds(f, qa, qb, fv, 64);
-        if (k == 0) o = r;
-        else o2 = r;
+    while (true) {
+      if (lane == 0) {
+        const int np = q.passes();
+        ctl[0] = np;
+        for (int k = 0; k < np; ++k) q.interval(k, &iv[k][0], &iv[k][1]);
       }
-      st = q.consume(o, o2);
+      __syncthreads();
+      const int np = ctl[0];
+      if (lane < 21 * np) {
+        const int pass = lane >= 21, e = lane - 21 * pass;
+        const double a = iv[pass][0], b = iv[pass][1];
+        vals[pass][e] = f(qk21_abscissa(e, 0.5 * (a + b), 0.5 * (b - a)));
+      }
+      __syncthreads();
+      if (lane == 0) {
+        const QK21Out o = qk21_sum([&](int e, double) { return vals[0][e]; }, iv[0][0], iv[0][1], fvs, 1);
+        QK21Out o2{};
+        if (np == 2) o2 = qk21_sum([&](int e, double) { return vals[1][e]; }, iv[1][0], iv[1][1], fvs, 1);
+        ctl[1] = q.consume(o, o2);
+      }
+      __syncthreads();
+      st = ctl[1];
+      if (st != kQagsRunning) break;
     }
-    const double Q = q.result / ft.A;
-    flux[ft.out] = ft.factor * Q * Q / ft.k;
-    const int status = st == kQagsDone ? q.status() : 99;
-    if (ier_out) ier_out[ft.out] = status;
-    atomicAdd(&stats[0], (unsigned long long)q.neval());
-    if (status != 0) {
-      atomicAdd(&stats[1], 1ull);
-      ts.failed(ft);
+    if (lane == 0) {
+      const double Q = q.result / ft.A;
+      flux[ft.out] = ft.factor * Q * Q / ft.k;
+      const int status = st == kQagsDone ? q.status() : 99;
+      if (ier_out) ier_out[ft.out] = status;
+      atomicAdd(&stats[0], (unsigned long long)q.neval());
+      if (status != 0) {
+        atomicAdd(&stats[1], 1ull);
+        ts.failed(ft);
+      }
     }
+    __syncthreads();
   }
 }
 
@@ -764,7 +787,7 @@ __global__ __launch_bounds__(BS) void k_scan_top(long long* __restrict__ part, i
 template <int BS = kScanBlock>
 __global__ __launch_bounds__(BS) void k_scan_apply(const int* __restrict__ cnt, long n,
                                                            const long long* __restrict__ part,
-                                                           int* __restrict__ off)
+                                                           long long* __restrict__ off)
 {
   __shared__ long long s[BS];
   const long i = (long)blockIdx.x * BS + threadIdx.x;
@@ -777,7 +800,7 @@ __global__ __launch_bounds__(BS) void k_scan_apply(const int* __restrict__ cnt, 
     s[threadIdx.x] += t;
     __syncthreads();
   }
-  if (i < n) off[i] = (int)(part[blockIdx.x] + s[threadIdx.x] - v);
+  if (i < n) off[i] = part[blockIdx.x] + s[threadIdx.x] - v;
 }
 
 inline FluxParams flux_params(const upc_ctx* c, int mirror_ny = 0)

@@ -16,6 +16,7 @@
 #include "upc_qags.h"
 #include "upc_flux.h"
 
+#include <algorithm>
 #include <climits>
 
 namespace upc {
@@ -50,13 +51,12 @@ __global__ void k_rows(const double* __restrict__ pts, int npts, int nrows, Flux
 // for the others.
 __global__ void k_flux_point_rows(int nrows, FluxParams fp, const double* __restrict__ row_k,
                                   const double* __restrict__ row_logd, const int* __restrict__ row_cnt,
-                                  const int* __restrict__ row_off, double* __restrict__ flux,
-                                  int* __restrict__ tasks, double2* __restrict__ task_bk)
+                                  const long long* __restrict__ row_off, double* __restrict__ flux,
+                                  TaskRec* __restrict__ recs)
 {
   const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= (long)nrows * kNB1) return;
-  point_row_node((int)(gid / kNB1), (int)(gid % kNB1), fp, row_k, row_logd, row_cnt, row_off, flux, tasks,
-                 task_bk);
+  point_row_node(gid / kNB1, (int)(gid % kNB1), fp, row_k, row_logd, row_cnt, row_off, flux, recs);
 }
 
 // ---------------------------------------------------------------------------
@@ -73,33 +73,38 @@ static int qags_persistent_blocks()
   return ncu * UPC_QAGS_MIN_WAVES;  // 256-lane workgroups: one wave per SIMD each
 }
 
-// Work buffers for npts points and their flux rows (at most 2 npts).  QAGS
-// task ids are row * 128 + node in 32 bits and row offsets are int, so more
-// than 2^24 - 1 rows are refused here, before anything is allocated or run.
-static int ensure_work(upc_ctx* c, size_t npts, size_t rows)
+// QAGS nodes of one flux row are at most 97: node j has b_j >= bmin e^{j logd}
+// with bmin = 0.05 R and logd >= ln(5R / bmin) / 120, so b_j <= 2R needs
+// j <= 120 ln 40 / ln 100 = 96.09.
+constexpr int kMaxQagsNodesPerRow = 98;
+
+// Work buffers for npts points and their flux rows (at most 2 npts); task
+// records, offsets and overflow lists are 64-bit.  The point-charge flux has
+// no QAGS tasks, so its task buffers stay minimal; the per-point form-factor
+// call needs `tasks` of them (its point indices and first passes).
+static int ensure_work(upc_ctx* c, size_t npts, size_t rows, size_t tasks = 0)
 {
   LumiWork& w = c->work;
-  if (rows * 128 >= (size_t(1) << 31))
-    return set_error("too many flux rows for 32-bit QAGS task ids (" + std::to_string(rows) +
-                     " rows, at most 16777215)"),
-           UPC_EINVAL;
-  if (rows <= w.rows_cap && npts <= w.points_cap) return 0;
+  const size_t need_tasks = std::max<size_t>(
+      {tasks, c->cfg.flux_point ? (size_t)1 : rows * kMaxQagsNodesPerRow, (size_t)1});
+  if (rows <= w.rows_cap && npts <= w.points_cap && need_tasks <= w.tasks_cap) return 0;
+  rows = std::max(rows, w.rows_cap);
+  npts = std::max(npts, w.points_cap);
+  const size_t ntask = std::max(need_tasks, w.tasks_cap);
   free_work(c);
   UPC_HIP(hipMalloc(&w.d_row_k, sizeof(double) * rows));
   UPC_HIP(hipMalloc(&w.d_row_logd, sizeof(double) * rows));
   UPC_HIP(hipMalloc(&w.d_row_cnt, sizeof(int) * rows));
-  UPC_HIP(hipMalloc(&w.d_row_off, sizeof(int) * rows));
+  UPC_HIP(hipMalloc(&w.d_row_off, sizeof(long long) * rows));
   UPC_HIP(hipMalloc(&w.d_flux, sizeof(double) * rows * kNB1));
-  UPC_HIP(hipMalloc(&w.d_tasks, sizeof(int) * rows * kNB1));
-  UPC_HIP(hipMalloc(&w.d_task_bk, sizeof(double2) * rows * kNB1));
-  UPC_HIP(hipMalloc(&w.d_first, sizeof(double4) * rows * kNB1));
-  UPC_HIP(hipMalloc(&w.d_overflow, sizeof(int) * rows * kNB1));
+  UPC_HIP(hipMalloc(&w.d_tasks, sizeof(TaskRec) * ntask));
+  UPC_HIP(hipMalloc(&w.d_first, sizeof(double4) * ntask));
+  UPC_HIP(hipMalloc(&w.d_overflow, sizeof(long long) * ntask));
   UPC_HIP(hipMalloc(&w.d_counters, sizeof(unsigned long long) * 8));
   UPC_HIP(hipMalloc(&w.d_points, sizeof(double) * 4 * npts));
   UPC_HIP(hipMalloc(&w.d_part, sizeof(long long) * ((rows + 1023) / 1024 + 1)));
-  UPC_HIP(hipMalloc(&w.d_big, sizeof(QagsBigArrays) * kQagsBigLanes));
   w.rows_cap = rows;
-  w.tasks_cap = rows * kNB1;
+  w.tasks_cap = ntask;
   w.points_cap = npts;
   return 0;
 }
@@ -107,9 +112,8 @@ static int ensure_work(upc_ctx* c, size_t npts, size_t rows)
 void free_work(upc_ctx* c)
 {
   LumiWork& w = c->work;
-  void* ptrs[] = {w.d_row_k,   w.d_row_logd, w.d_row_cnt, w.d_row_off, w.d_flux, w.d_tasks, w.d_task_bk,
-                  w.d_first,
-                  w.d_overflow, w.d_counters, w.d_points, w.d_part, w.d_big};
+  void* ptrs[] = {w.d_row_k,    w.d_row_logd, w.d_row_cnt, w.d_row_off, w.d_flux, w.d_tasks,
+                  w.d_first,    w.d_overflow, w.d_counters, w.d_points, w.d_part};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   w = LumiWork{};
@@ -142,19 +146,19 @@ static int run_flux(upc_ctx* c, int npts, int mirror_ny, hipStream_t s)
   hipLaunchKernelGGL(k_scan_apply<>, dim3(nparts), dim3(kScanBlock), 0, s, w.d_row_cnt, (long)nrows, w.d_part,
                      w.d_row_off);
   hipLaunchKernelGGL(k_flux_point_rows, dim3(nblk((long)nrows * kNB1, 256)), dim3(256), 0, s, nrows, fp,
-                     w.d_row_k, w.d_row_logd, w.d_row_cnt, w.d_row_off, w.d_flux, w.d_tasks, w.d_task_bk);
+                     w.d_row_k, w.d_row_logd, w.d_row_cnt, w.d_row_off, w.d_flux, (TaskRec*)w.d_tasks);
   UPC_HIP(hipGetLastError());
   }
   if (!c->cfg.flux_point) {
     PhaseScope ph(c, s, UPC_PHASE_QAGS);
-    GridTask ts{w.d_tasks, w.d_task_bk, fp, c->d_ff};
+    GridTask ts{(const TaskRec*)w.d_tasks, fp, c->d_ff};
     hipLaunchKernelGGL((k_flux_first<GridTask>), dim3(qags_persistent_blocks()), dim3(256), 0, s, ts,
                        w.d_counters + 4, w.d_first);
     hipLaunchKernelGGL((k_flux_qags<kQagsCap, GridTask>), dim3(qags_persistent_blocks()), dim3(256), 0, s, ts,
                        w.d_counters + 4, w.d_counters + 5, w.d_flux, (int*)nullptr, w.d_counters,
                        w.d_overflow, c->qags_cap, (const double4*)w.d_first);
-    hipLaunchKernelGGL((k_flux_qags_big<GridTask>), dim3(kQagsBigLanes / 64), dim3(64), 0, s, ts, w.d_overflow,
-                       w.d_counters, w.d_flux, (int*)nullptr, (QagsBigArrays*)w.d_big, kQagsBigLanes);
+    hipLaunchKernelGGL((k_flux_qags_wave<GridTask>), dim3(kQagsWaveBlocks), dim3(64), 0, s, ts, w.d_overflow,
+                       w.d_counters, w.d_flux, (int*)nullptr);
     UPC_HIP(hipGetLastError());
   }
   return 0;
@@ -393,8 +397,8 @@ int enqueue_flux_form(upc_ctx* c, const double* d_b, const double* d_k, int n, d
   hipLaunchKernelGGL((k_flux_qags<kQagsCap, PointTask>), dim3(qags_persistent_blocks()), dim3(256), 0, s, ts,
                      w.counters + 4, w.counters + 5, d_out, d_ier, w.counters, w.overflow, c->qags_cap,
                      (const double4*)w.first);
-  hipLaunchKernelGGL((k_flux_qags_big<PointTask>), dim3(w.nbig / 64), dim3(64), 0, s, ts, w.overflow, w.counters,
-                     d_out, d_ier, w.big, w.nbig);
+  hipLaunchKernelGGL((k_flux_qags_wave<PointTask>), dim3(kQagsWaveBlocks), dim3(64), 0, s, ts, w.overflow,
+                     w.counters, d_out, d_ier);
   UPC_HIP(hipGetLastError());
   return 0;
 }
@@ -403,12 +407,11 @@ int launch_flux_form(upc_ctx* c, const double* d_b, const double* d_k, int n, do
                      hipStream_t s)
 {
   if (n <= 0) return 0;
-  // the per-point QAGS uses the task arrays (rows x 120 ints) for its n indices
-  int rc = ensure_work(c, 1, (size_t)(n + kNB1 - 1) / kNB1);
+  // the per-point QAGS keeps its n point indices in the task buffer
+  int rc = ensure_work(c, 1, 1, (size_t)n);
   if (rc) return rc;
   LumiWork& w = c->work;
-  const FluxFormBufs bufs{w.d_tasks, w.d_first, w.d_overflow, w.d_counters, (QagsBigArrays*)w.d_big,
-                          kQagsBigLanes};
+  const FluxFormBufs bufs{(int*)w.d_tasks, w.d_first, w.d_overflow, w.d_counters};
   rc = enqueue_flux_form(c, d_b, d_k, n, d_out, d_ier, bufs, s);
   if (rc) return rc;
   return read_stats(c, s);

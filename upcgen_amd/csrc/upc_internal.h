@@ -37,60 +37,31 @@ struct LumiWork {
   double* d_row_k = nullptr;      // photon energy of each flux row
   double* d_row_logd = nullptr;   // log step of the row's b grid
   int* d_row_cnt = nullptr;       // number of QAGS nodes in the row
-  int* d_row_off = nullptr;       // exclusive scan of d_row_cnt
+  long long* d_row_off = nullptr; // exclusive scan of d_row_cnt
   double* d_flux = nullptr;       // [rows][120]
   size_t tasks_cap = 0;
-  int* d_tasks = nullptr;         // row*128 + j
-  double2* d_task_bk = nullptr;   // (b, k) of each task
+  void* d_tasks = nullptr;        // QAGS task records (b, k, flux slot); per-point calls: point indices
   double4* d_first = nullptr;     // first QK21 pass of each task (result, abserr, resabs, resasc)
   unsigned long long* d_counters = nullptr;  // [0] task cursor, [1] qags bad, [2] evals, [3] overflow n
-  int* d_overflow = nullptr;      // task ids that exceeded the fast workspace
+  long long* d_overflow = nullptr;  // tasks that exceeded the fast workspace
   double* d_points = nullptr;     // (M, Y, dm, dy) per point
   long long* d_part = nullptr;    // per-block partial sums of the row-count scan
   size_t points_cap = 0;
-  void* d_big = nullptr;          // QAGS overflow workspace (kQagsBigLanes lanes)
 };
-
-constexpr int kQagsBigLanes = 256;
-
-struct QagsBigArrays;  // upc_qags.h
 
 // Buffers of one per-point form-factor flux launch (upc_flux_form).
 struct FluxFormBufs {
   int* idx;                      // [n] compacted indices of the points that need QAGS
   double4* first;                // [n] their first QK21 passes
-  int* overflow;                 // [n] tasks that outgrew the fast interval list
+  long long* overflow;           // [n] tasks that outgrew the fast interval list
   unsigned long long* counters;  // [8] evaluations, failed, overflow count, -, tasks, cursor
-  QagsBigArrays* big;            // [nbig] 1000-interval QAGS workspaces
-  int nbig;                      // a multiple of 64
 };
 
-// upc_events.hip
-int build_pt_tables(upc_ctx* c, double gtot, hipStream_t s);
-void free_events(upc_ctx* c);
-int launch_photon_pt(upc_ctx* c, const double* d_E, long long n, unsigned long long seed, double* d_out,
-                     hipStream_t s);
-int launch_generate_pairs(upc_ctx* c, const double* d_sum_ym, const double* d_sum_z, const double* d_sum_zs,
-                          const double* d_sum_zp, const double* d_ratio, const upc_event_opts* opts,
-                          unsigned long long seed, unsigned long long offset, long long n, double* d_p, int* d_pdg,
-                          int* d_pass, hipStream_t s);
-
-}  // namespace upc
-
-namespace upc {
+// One timed phase: the event pair recorded around it when timing is enabled.
 struct TimedRec {
   int phase;
   hipEvent_t beg, end;
 };
-// upc_events.hip
-int build_pt_tables(upc_ctx* c, double gtot, hipStream_t s);
-void free_events(upc_ctx* c);
-int launch_photon_pt(upc_ctx* c, const double* d_E, long long n, unsigned long long seed, double* d_out,
-                     hipStream_t s);
-int launch_generate_pairs(upc_ctx* c, const double* d_sum_ym, const double* d_sum_z, const double* d_sum_zs,
-                          const double* d_sum_zp, const double* d_ratio, const upc_event_opts* opts,
-                          unsigned long long seed, unsigned long long offset, long long n, double* d_p, int* d_pdg,
-                          int* d_pass, hipStream_t s);
 
 }  // namespace upc
 
