@@ -127,6 +127,17 @@ int upc_flux_point(upc_ctx *ctx, const double *d_b, const double *d_k, int n, do
  * d_ier (may be NULL) receives the QAGS status per point. */
 int upc_flux_form(upc_ctx *ctx, const double *d_b, const double *d_k, int n, double *d_out,
                   int *d_ier, void *stream);
+/* upc_flux_form without any allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * kernels and results as upc_flux_form; its work buffers are one file-scope
+ * device array per device (11.5 MB, held until the process ends) instead of
+ * the context's.  At most 262,144 points per call (UPC_EINVAL beyond).  Calls
+ * that overlap on one device race on that array; the context's QAGS counters
+ * (upc_ctx_info) are not updated.  The current device must be the
+ * context's. */
+int upc_flux_form_noalloc(upc_ctx *ctx, const double *d_b, const double *d_k, int n, double *d_out,
+                          int *d_ier, void *stream);
 /* J1(x), x >= 0, exactly as the form-factor flux integrand evaluates it on
  * the device (role of gsl_sf_bessel_J1 in fluxForm, src/UpcCrossSection.cpp:189).
  * No context needed. */

@@ -101,6 +101,7 @@ SIGNATURES = {
     "upc_ctx_set_qags_cap": (_i, [_vp, _i]),
     "upc_flux_point": (_i, [_vp, _vp, _vp, _i, _vp, _vp]),
     "upc_flux_form": (_i, [_vp, _vp, _vp, _i, _vp, _vp, _vp]),
+    "upc_flux_form_noalloc": (_i, [_vp, _vp, _vp, _i, _vp, _vp, _vp]),
     "upc_bessel_j1": (_i, [_vp, _i, _vp, _vp]),
     "upc_form_factor": (_i, [_vp, _vp, _i, _i, _vp, _vp]),
     "upc_breakup_prob": (_i, [_vp, _vp, _i, _i, _i, _vp, _vp]),

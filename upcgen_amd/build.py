@@ -18,7 +18,7 @@ LIB = os.path.join(HERE, "libupcgen_amd.so")
 EXE = os.path.join(HERE, "upcgen")
 
 HIP_SOURCES = ["upc_capi.hip", "upc_tables.hip", "upc_flux_lumi.hip", "upc_xsec.hip", "upc_mfma.hip",
-               "upc_surrogate.hip", "upc_events.hip", "upc_batch.hip"]
+               "upc_surrogate.hip", "upc_events.hip", "upc_batch.hip", "upc_noalloc.hip"]
 CXX_SOURCES = ["UpcCrossSection.cpp", "UpcGenerator.cpp"]
 
 ARCH = os.environ.get("UPC_OFFLOAD_ARCH", "gfx950")
