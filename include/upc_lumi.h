@@ -124,7 +124,12 @@ int upc_ctx_timings(upc_ctx *ctx, upc_timings *h_out, int reset);
 int upc_flux_point(upc_ctx *ctx, const double *d_b, const double *d_k, int n, double *d_out,
                    void *stream);
 /* UpcCrossSection::fluxForm(b, k)             src/UpcCrossSection.cpp:194-218
- * d_ier (may be NULL) receives the QAGS status per point. */
+ * d_ier (may be NULL) receives the QAGS status per point: dqagse's ier
+ * (0 = converged, 1..5), 99 = still unfinished at the 1000-interval limit
+ * (not reached by QUADPACK's own stopping rules), or 100 = a node of a QK21
+ * pass lies past the last form-factor knot (t > 1.999998 GeV^2), where the
+ * reference's gsl_spline_eval raises GSL_EDOM and its error handler ends the
+ * program: the integral stops after that pass and its flux is NaN. */
 int upc_flux_form(upc_ctx *ctx, const double *d_b, const double *d_k, int n, double *d_out,
                   int *d_ier, void *stream);
 /* upc_flux_form without any allocation.  No counterpart in the reference: it

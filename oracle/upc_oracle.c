@@ -483,9 +483,12 @@ L100:
   *abserr = fmax(*abserr, 5.0 * epmach * fabs(*result));
 }
 
-int orc_qags_trace(orc_fn f, void *p, double a, double b, double epsabs, double epsrel,
-                   int limit, double *result, double *abserr, int *neval, int *last_out,
-                   double *oal, double *obl, double *orl, double *oel)
+/* stop_nan != 0: the integral ends with ier ORC_QAGS_EDOM after the first
+ * QK21 pass (or bisection step) whose value is NaN; neval counts that pass.
+ * With stop_nan == 0 this is dqagse as QUADPACK runs it. */
+static int qags_core(orc_fn f, void *p, double a, double b, double epsabs, double epsrel, int limit,
+                     int stop_nan, double *result, double *abserr, int *neval, int *last_out,
+                     double *oal, double *obl, double *orl, double *oel)
 {
   const double epmach = DBL_EPSILON, uflow = DBL_MIN, oflow = DBL_MAX;
   double *alist = (double *)malloc(sizeof(double) * (limit + 1));
@@ -511,6 +514,7 @@ int orc_qags_trace(orc_fn f, void *p, double a, double b, double epsabs, double 
   rlist[1] = *result;
   elist[1] = *abserr;
   iord[1] = 1;
+  if (stop_nan && *result != *result) { ier = ORC_QAGS_EDOM; goto L140; }
   if (*abserr <= 100.0 * epmach * defabs && *abserr > errbnd) ier = 2;
   if (limit == 1) ier = 1;
   if (ier != 0 || (*abserr <= errbnd && *abserr != resabs) || *abserr == 0.0) goto L140;
@@ -542,6 +546,7 @@ int orc_qags_trace(orc_fn f, void *p, double a, double b, double epsabs, double 
     qk21(f, p, a2, b2, &area2, &error2, &resabs, &defab2);
     area12 = area1 + area2;
     erro12 = error1 + error2;
+    if (stop_nan && area12 != area12) { *result = area12; ier = ORC_QAGS_EDOM; goto L140; }
     errsum = errsum + erro12 - errmax;
     area = area + area12 - rlist[maxerr];
     if (!(defab1 == error1 || defab2 == error2)) {
@@ -663,11 +668,17 @@ L999:
   return ier;
 }
 
+int orc_qags_trace(orc_fn f, void *p, double a, double b, double epsabs, double epsrel,
+                   int limit, double *result, double *abserr, int *neval, int *last_out,
+                   double *oal, double *obl, double *orl, double *oel)
+{
+  return qags_core(f, p, a, b, epsabs, epsrel, limit, 0, result, abserr, neval, last_out, oal, obl, orl, oel);
+}
+
 int orc_qags(orc_fn f, void *p, double a, double b, double epsabs, double epsrel, int limit,
              double *result, double *abserr, int *neval, int *last)
 {
-  return orc_qags_trace(f, p, a, b, epsabs, epsrel, limit, result, abserr, neval, last,
-                        NULL, NULL, NULL, NULL);
+  return qags_core(f, p, a, b, epsabs, epsrel, limit, 0, result, abserr, neval, last, NULL, NULL, NULL, NULL);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1063,10 +1074,13 @@ double orc_flux_form(const orc_ctx *c, double b, double k, int *neval, int *ier)
   ff_par fp = {c, b, k, c->g1, 0};
   double res, err;
   int nev = 0, last = 0;
-  const int e = orc_qags(flux_form_integrand, &fp, 0., 10., c->qags_epsabs, c->qags_epsrel, 1000, &res, &err,
-                         &nev, &last);
+  /* A node past the last form-factor knot is where the reference's
+   * gsl_spline_eval raises GSL_EDOM and its default handler ends the
+   * program; the integrand returns NaN there and the integral stops. */
+  const int e = qags_core(flux_form_integrand, &fp, 0., 10., c->qags_epsabs, c->qags_epsrel, 1000, 1, &res,
+                          &err, &nev, &last, NULL, NULL, NULL, NULL);
   if (neval) *neval = nev;
-  if (ier) *ier = e ? e : (fp.err ? 100 : 0);
+  if (ier) *ier = e;
   const double Q = res / c->p.A;
   return c->factor * Q * Q / k;
 }

@@ -42,6 +42,10 @@ double orc_bessel_i1_as(double x);     /* A&S 9.8.3/9.8.4 I1 (TMath::BesselI1 ro
 double orc_simpson(int n, const double *v, double h);
 
 typedef double (*orc_fn)(double x, void *p);
+/* ier of a form-factor flux integral stopped at its first QK21 pass with a
+ * NaN value: a node past the last form-factor knot, where the reference's
+ * gsl_spline_eval raises GSL_EDOM (its default handler ends the program). */
+#define ORC_QAGS_EDOM 100
 /* QUADPACK dqagse (QK21 + Wynn epsilon).  Returns ier (0 = success). */
 int orc_qags(orc_fn f, void *p, double a, double b, double epsabs, double epsrel,
              int limit, double *result, double *abserr, int *neval, int *last);

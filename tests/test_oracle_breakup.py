@@ -22,7 +22,8 @@ flux integrand itself (the oracle's fluxFormIntegrand, the same Python
 callable for both) the oracle's dqagse must return the same result, error,
 evaluation count, status and final interval lists bit for bit, over (b, k)
 nodes of the 1001 x 121 grid's rows, including ones with many intervals and
-the ones whose nodes leave the form-factor table (NaN, ier 2).
+the ones whose nodes leave the form-factor table (NaN, ier 2); the oracle's
+flux integral stops at the first pass that holds such a node.
 """
 import math
 import os
@@ -190,8 +191,15 @@ def test_qags_trace_matches_quadpack_on_flux_integrand(tables, pick):
     def f(q):
         return o.flux_integrand(b, k, q)
 
+    calls = []
+
+    def f_logged(q):
+        v = f(q)
+        calls.append(v)
+        return v
+
     res, err, nev, ier, (al, bl, rl, el) = O.qags(f, 0.0, 10.0, 1e-4, 1e-4)
-    s = si.quad(f, 0.0, 10.0, epsabs=1e-4, epsrel=1e-4, limit=1000, full_output=1)
+    s = si.quad(f_logged, 0.0, 10.0, epsabs=1e-4, epsrel=1e-4, limit=1000, full_output=1)
     info = s[2]
     n = info["last"]
     print(f"b {b:.4f} k {k:.4g}: result {res!r} neval {nev} intervals {n} ier {ier}")
@@ -200,6 +208,17 @@ def test_qags_trace_matches_quadpack_on_flux_integrand(tables, pick):
     assert same(np.array([res, err]), np.array([s[0], s[1]]))
     for mine, key in ((al, "alist"), (bl, "blist"), (rl, "rlist"), (el, "elist")):
         assert same(mine, info[key][:n]), key
-    # the oracle's flux value and status are this QAGS run's
+    # The oracle's flux value and status are this QAGS run's, except that an
+    # integral stops after the first QK21 pass holding a NaN node (past the
+    # last form-factor knot, the reference's GSL_EDOM): dqagse evaluates the
+    # whole range (pass 0), then two passes per bisection step, 21 nodes
+    # each, in the order scipy called the integrand.
     v, nev2, ier2 = o.flux_form(b, k)
-    assert nev2 == nev and (ier2 != 0) == (ier != 0 or np.isnan(res))
+    assert len(calls) == nev
+    bad = np.flatnonzero(np.isnan(np.array(calls)))
+    if bad.size == 0:
+        assert nev2 == nev and (ier2 != 0) == (ier != 0)
+        assert v == v
+    else:
+        p = int(bad[0]) // 21
+        assert nev2 == 21 + 42 * ((p + 1) // 2) and ier2 == 100 and np.isnan(v)

@@ -24,6 +24,13 @@ namespace upc {
 
 enum QagsStatus { kQagsRunning = 0, kQagsDone = 1, kQagsOverflow = 2 };
 
+// ier of an integral stopped at its first QK21 pass with a NaN integrand
+// value.  In the flux integrand NaN marks a node past the last form-factor
+// knot, where the reference's gsl_spline_eval raises GSL_EDOM and its
+// default error handler ends the program: the integral stops there rather
+// than bisecting NaN values up to the 1000-interval limit.
+constexpr int kQagsEdom = 100;
+
 constexpr double kQagsA = 0.0, kQagsB = 10.0;  // integration range of every flux integral
 constexpr double kQagsEps = 1e-4;               // epsabs = epsrel of the reference
 constexpr int kQagsLimit = 1000;                // gsl_integration_qags limit of the reference
@@ -245,10 +252,14 @@ struct Qags {
       result = o.result;
       abserr = o.abserr;
       defabs = o.resabs;
+      last = 1;
+      if (result != result) {
+        ier = kQagsEdom;
+        return kQagsDone;
+      }
       const double resasc0 = o.resasc;
       const double dres = fabs(result);
       errbnd = fmax(kQagsEps, kQagsEps * dres);
-      last = 1;
       s.set_iv(1, cur);
       s.set_rl(1, result);
       s.set_el(1, abserr);
@@ -283,6 +294,11 @@ struct Qags {
     const double area2 = o2.result, error2 = o2.abserr, defab2 = o2.resasc;
     const double area12 = area1 + area2;
     const double erro12 = error1 + error2;
+    if (area12 != area12) {
+      result = area12;
+      ier = kQagsEdom;
+      return kQagsDone;
+    }
     const double rmax = s.rl(maxerr);
     errsum = errsum + erro12 - errmax;
     area = area + area12 - rmax;
@@ -376,16 +392,8 @@ struct Qags {
 
   __device__ int neval() const { return 42 * last - 21; }
 
-  // Status of a finished integral as the reference sees it: dqagse's ier,
-  // or 100 when an integrand value was NaN (the form-factor spline's
-  // GSL_EDOM) without an ier.  A NaN value makes area and errsum NaN for
-  // good, and the result NaN unless an earlier extrapolation is returned.
-  __device__ int status() const
-  {
-    if (ier != 0) return ier;
-    const bool nan = result != result || (last > 1 && (errsum != errsum || area != area));
-    return nan ? 100 : 0;
-  }
+  // Status of a finished integral: dqagse's ier, or kQagsEdom.
+  __device__ int status() const { return ier; }
 };
 
 }  // namespace upc
