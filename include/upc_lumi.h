@@ -136,7 +136,7 @@ int upc_flux_form(upc_ctx *ctx, const double *d_b, const double *d_k, int n, dou
  * is for callers that cannot allocate device memory (code under HIP graph
  * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
  * kernels and results as upc_flux_form; its work buffers are one file-scope
- * device array per device (11.5 MB, held until the process ends) instead of
+ * device array per device (3 MB, held until the process ends) instead of
  * the context's.  At most 262,144 points per call (UPC_EINVAL beyond).  Calls
  * that overlap on one device race on that array; the context's QAGS counters
  * (upc_ctx_info) are not updated.  The current device must be the
@@ -320,9 +320,11 @@ int upc_sur_features(const upc_config *cfg, const double *d_M, const double *d_Y
 int upc_sur_forward(upc_sur *sur, int net, const float *d_X, int B, float *d_pred, void *stream);
 /* Gradient of mean((pred - t)^2) w.r.t. all parameters of `net` (param_count floats). */
 int upc_sur_grad(upc_sur *sur, int net, const float *d_X, const float *d_T, int B, float *d_grad, void *stream);
-/* Adam training of both networks on n rows (mini-batches drawn with Philox);
- * use_graph: one captured step replayed `steps` times.  h_loss (optional):
- * [2][steps] mean squared error per step. */
+/* Adam training of both networks on n rows (mini-batches drawn with Philox),
+ * from their current weights with fresh moments (so a second call fine-tunes);
+ * learning rate lr decaying by a cosine to lr/100 over `steps`.  use_graph:
+ * one captured step replayed `steps` times.  h_loss (optional): [2][steps]
+ * mean squared error per step. */
 int upc_sur_train(upc_sur *sur, const float *d_X, const float *d_T, long long n, int steps, int batch, double lr,
                   int use_graph, void *stream, float *h_loss);
 /* Attach a trained surrogate to a context for upc_lumi_grid(UPC_FORM_SURROGATE):

@@ -104,7 +104,7 @@ def test_batch_autotune(batch_and_refs):
     assert t.qags_blocks_per_cu in list(t.qags_candidates)
     assert t.lumi_waves in list(t.lumi_candidates)
     assert all(ms > 0 for ms in t.qags_ms) and all(ms > 0 for ms in t.lumi_ms)
-    for qb in (2, 4):
+    for qb in (1, 3):
         for lw in (2, 8):
             b.set_tune(qb, lw)
             got = _grids(b.run(graph=True), cfgs)

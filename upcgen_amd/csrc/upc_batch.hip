@@ -32,7 +32,7 @@ struct BatchJob {
 };
 
 constexpr int kLumiWaveChoices[3] = {2, 4, 8};
-constexpr int kQagsBlockChoices[3] = {2, 3, 4};  // workgroups of 256 per CU
+constexpr int kQagsBlockChoices[3] = {1, 2, 3};  // workgroups of 256 per CU (3 fit its LDS)
 
 __device__ __forceinline__ int find_job(const int* __restrict__ start, int njobs, long idx)
 {
@@ -91,6 +91,7 @@ __global__ void k_batch_point_rows(const BatchJob* __restrict__ jobs, const int*
 
 // QAGS task source over all jobs: task records of the concatenated flux rows.
 struct BatchTask {
+  static constexpr bool kSharedFF = false;  // each job has its nucleus' table (FluxTask::ff)
   const TaskRec* __restrict__ recs;
   const BatchJob* __restrict__ jobs;
   const int* __restrict__ row0;
@@ -152,7 +153,6 @@ struct upc_batch {
   long long ntasks = 0;          // QAGS tasks of every job (counted at creation)
   long long work_bytes = 0;      // device memory of the buffers below
   TaskRec* d_tasks = nullptr;
-  double4* d_first = nullptr;  // first QK21 pass of each task (k_flux_first)
   long long* d_overflow = nullptr;
   unsigned long long* d_counters = nullptr;  // [0] evals [1] bad [2] overflow n [4] tasks [5] cursor
   unsigned long long* d_job_failed = nullptr;  // [njobs] QAGS integrals with ier != 0 per job
@@ -182,7 +182,7 @@ void free_batch(upc_batch* b)
   if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
   void* ptrs[] = {b->d_jobs, b->d_pt0, b->d_row0, b->d_blk0[0], b->d_blk0[1], b->d_blk0[2], b->d_pts,
                   b->d_row_k, b->d_row_logd, b->d_row_cnt, b->d_row_off, b->d_part, b->d_flux, b->d_tasks,
-                  b->d_first, b->d_overflow, b->d_counters, b->d_job_failed};
+                  b->d_overflow, b->d_counters, b->d_job_failed};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (upc_ctx* c : b->ctx) upc_ctx_destroy(c);
@@ -207,11 +207,9 @@ int enqueue(upc_batch* b, hipStream_t s)
                      nj, b->nrows, b->d_row_k, b->d_row_logd, b->d_row_cnt, b->d_row_off, b->d_flux, b->d_tasks);
   BatchTask ts{b->d_tasks, b->d_jobs, b->d_row0, nj, b->d_job_failed};
   const int qblocks = b->ncu * kQagsBlockChoices[b->qags_choice];
-  hipLaunchKernelGGL((k_flux_first<BatchTask>), dim3(b->ncu * kQagsBlockChoices[2]), dim3(256), 0, s, ts,
-                     b->d_counters + 4, b->d_first);
-  hipLaunchKernelGGL((k_flux_qags<kQagsCap, BatchTask>), dim3(qblocks), dim3(256), 0, s, ts, b->d_counters + 4,
-                     b->d_counters + 5, b->d_flux, (int*)nullptr, b->d_counters, b->d_overflow,
-                     kQagsCap, (const double4*)b->d_first);
+  hipLaunchKernelGGL((k_flux_qags<kQagsCap, BatchTask>), dim3(qblocks), dim3(kQagsBlock), 0, s, ts,
+                     b->d_counters + 4, b->d_counters + 5, b->d_flux, (int*)nullptr, b->d_counters, b->d_overflow,
+                     kQagsCap);
   hipLaunchKernelGGL((k_flux_qags_wave<BatchTask>), dim3(kQagsWaveBlocks), dim3(64), 0, s, ts, b->d_overflow,
                      b->d_counters, b->d_flux, (int*)nullptr);
   const int li = b->lumi_choice, w = kLumiWaveChoices[li];
@@ -335,7 +333,7 @@ int upc_batch_create(const upc_config* cfgs, int njobs, void* stream, upc_batch*
     free_batch(b);
     return rc;
   }
-  // The QAGS task buffers (records, first passes, overflow list) are sized by
+  // The QAGS task buffers (records, overflow list) are sized by
   // the batch's task count, which the row kernels and the scan give once here.
   auto size_tasks = [&]() -> int {
     const int nj = (int)b->jobs.size();
@@ -354,9 +352,8 @@ int upc_batch_create(const upc_config* cfgs, int njobs, void* stream, upc_batch*
     b->ntasks = (long long)total;
     const size_t n = total > 0 ? (size_t)total : 1;
     UPC_HIP(hipMalloc(&b->d_tasks, sizeof(TaskRec) * n));
-    UPC_HIP(hipMalloc(&b->d_first, sizeof(double4) * n));
     UPC_HIP(hipMalloc(&b->d_overflow, sizeof(long long) * n));
-    b->work_bytes += (long long)((sizeof(TaskRec) + sizeof(double4) + sizeof(long long)) * n);
+    b->work_bytes += (long long)((sizeof(TaskRec) + sizeof(long long)) * n);
     return 0;
   };
   if (int rc = size_tasks()) {
@@ -436,15 +433,14 @@ int upc_batch_autotune(upc_batch* b, void* stream, upc_tune* h_out)
   const int nj = (int)b->jobs.size();
   BatchTask ts{b->d_tasks, b->d_jobs, b->d_row0, nj, b->d_job_failed};
   for (int i = 0; i < 3; ++i) {
-    // QAGS alone (the first passes of the warm run stay valid: they depend on
-    // the task list only): clear the counters except the task total, then time
+    // QAGS alone: clear the counters except the task total, then time
     hipLaunchKernelGGL(k_batch_zero, dim3(1), dim3(64), 0, s, b->d_counters, 4);
     hipLaunchKernelGGL(k_batch_zero, dim3(1), dim3(64), 0, s, b->d_counters + 5, 1);
     UPC_HIP(hipEventRecord(e0, s));
     const int qblocks = b->ncu * kQagsBlockChoices[i];
-    hipLaunchKernelGGL((k_flux_qags<kQagsCap, BatchTask>), dim3(qblocks), dim3(256), 0, s, ts, b->d_counters + 4,
-                       b->d_counters + 5, b->d_flux, (int*)nullptr, b->d_counters, b->d_overflow,
-                     kQagsCap, (const double4*)b->d_first);
+    hipLaunchKernelGGL((k_flux_qags<kQagsCap, BatchTask>), dim3(qblocks), dim3(kQagsBlock), 0, s, ts,
+                       b->d_counters + 4, b->d_counters + 5, b->d_flux, (int*)nullptr, b->d_counters, b->d_overflow,
+                       kQagsCap);
     UPC_HIP(hipEventRecord(e1, s));
     UPC_HIP(hipEventSynchronize(e1));
     float ms = 0;

@@ -64,7 +64,7 @@ __device__ __forceinline__ double clenshaw(const double (&c)[N], double u)
 {
   double b1 = 0.0, b2 = 0.0;
   const double u2 = 2.0 * u;
-#pragma unroll 4
+#pragma unroll
   for (int k = N - 1; k >= 1; --k) {
     const double t = fma(u2, b1, c[k] - b2);
     b2 = b1;
@@ -461,13 +461,14 @@ __device__ __forceinline__ double qk21_abscissa(int e, double centr, double hlgt
   return e == 0 ? centr : (side ? centr + absc : centr - absc);
 }
 
-// dqk21 as a rolled loop over the 21 nodes: the same operations in the same
-// order as qk21() above, with the 20 off-centre values parked at
-// fv[s * stride] (s = 0..19) until the mean deviation needs them.  fx(e, x)
-// gives the integrand at node e, abscissa x (evaluated there, or read from
-// where other lanes evaluated it).
-template <class FX>
-__device__ __forceinline__ QK21Out qk21_sum(const FX& fx, double a, double b, double* fv, int stride)
+// dqk21 as a rolled loop over the 21 nodes (e = 0 the centre, e = 1..10
+// the Gauss pairs, e = 11..20 the Kronrod-only pairs; qk21_abscissa): the
+// same operations in the same order as qk21() above.  fx(e, x) gives the
+// integrand at node e, abscissa x; gx(j, side) gives back the value of
+// pair j (kXgk index), side 0 (centr - absc) or 1 (centr + absc), for the
+// mean deviation once the Kronrod mean is known.
+template <class FX, class GX>
+__device__ __forceinline__ QK21Out qk21_core(const FX& fx, const GX& gx, double a, double b)
 {
   constexpr double epmach = DBL_EPSILON, uflow = DBL_MIN;
   const double centr = 0.5 * (a + b);
@@ -489,8 +490,6 @@ __device__ __forceinline__ QK21Out qk21_sum(const FX& fx, double a, double b, do
       f1 = fx_e;
     } else {
       const double f2 = fx_e;
-      fv[(2 * xi) * stride] = f1;
-      fv[(2 * xi + 1) * stride] = f2;
       const double fsum = f1 + f2;
       if (gauss) resg = resg + kWg[p] * fsum;
       resk = resk + kWgk[xi] * fsum;
@@ -500,8 +499,7 @@ __device__ __forceinline__ QK21Out qk21_sum(const FX& fx, double a, double b, do
   const double reskh = resk * 0.5;
   double rasc = kWgk[10] * fabs(fc - reskh);
 #pragma unroll
-  for (int j = 0; j < 10; ++j)
-    rasc = rasc + kWgk[j] * (fabs(fv[(2 * j) * stride] - reskh) + fabs(fv[(2 * j + 1) * stride] - reskh));
+  for (int j = 0; j < 10; ++j) rasc = rasc + kWgk[j] * (fabs(gx(j, 0) - reskh) + fabs(gx(j, 1) - reskh));
   QK21Out o;
   o.result = resk * hlgth;
   rabs = rabs * dhlgth;
@@ -522,7 +520,38 @@ __device__ __forceinline__ QK21Out qk21_sum(const FX& fx, double a, double b, do
   return o;
 }
 
-// dqk21 evaluating the integrand f at each node as it goes.
+// Node e of pair j (kXgk index), side 0 or 1 (the inverse of the mapping in
+// qk21_core).
+__device__ __forceinline__ int qk21_node(int j, int side) { return (j & 1) ? j + side : 11 + j + side; }
+
+// qk21_core evaluating the integrand: the 20 off-centre values are parked
+// at fv[(2 j + side) * stride] until the mean deviation needs them.
+template <class FX>
+__device__ __forceinline__ QK21Out qk21_sum(const FX& fx, double a, double b, double* fv, int stride)
+{
+  return qk21_core(
+      [&](int e, double x) {
+        const double v = fx(e, x);
+        if (e != 0) {
+          const int gauss = e <= 10;
+          const int p = gauss ? (e - 1) >> 1 : (e - 11) >> 1;
+          const int side = gauss ? (e - 1) & 1 : (e - 11) & 1;
+          const int xi = gauss ? 2 * p + 1 : 2 * p;
+          fv[(2 * xi + side) * stride] = v;
+        }
+        return v;
+      },
+      [&](int j, int side) { return fv[(2 * j + side) * stride]; }, a, b);
+}
+
+// qk21_core over values already evaluated: node e's value at v[e * stride].
+__device__ __forceinline__ QK21Out qk21_stored(const double* v, int stride, double a, double b)
+{
+  return qk21_core([&](int e, double) { return v[e * stride]; },
+                   [&](int j, int side) { return v[qk21_node(j, side) * stride]; }, a, b);
+}
+
+// dqk21 of f on [a, b] through qk21_sum.
 template <class F>
 __device__ __forceinline__ QK21Out qk21_lds(const F& f, double a, double b, double* fv, int stride)
 {

@@ -150,30 +150,68 @@ __device__ __forceinline__ void row_setup(const double* __restrict__ pts, int np
 //
 // The table's last knot is Q2min + (nQ2 - 1) dQ2 = 2 - 2e-6 (to rounding),
 // so for t in (last knot, Q2max) the reference's gsl_spline_eval is out of
-// range: GSL raises GSL_EDOM (its default handler aborts) and returns NaN.
-// The integrand returns that NaN, so the QAGS run continues exactly as
-// QUADPACK does with it, ends with a NaN result, and the integral is
-// counted as failed, where the reference process would have stopped.
+// range: GSL raises GSL_EDOM (its default handler ends the program).  The
+// integrand returns NaN there and the QAGS run stops after that QK21 pass
+// (kQagsEdom).
+//
+// The spline record is a dependent load from a 32 MB table: it is issued
+// first and J1 (independent ALU work) runs while it is in flight.  The
+// interval is the rounded estimate alone, as in spline_eval_near: within
+// rounding of a knot the neighbouring cubic agrees to a few ulp.
+//
+// BR fixes J1's branch: 0 lets J1 choose (x <= 8 or not), 1 and 2 take the
+// x <= 8 and x > 8 branches for a caller that has sorted its nodes by
+// flux_j1_arg.  The operations are the same in all three forms, so a value
+// does not depend on which one computed it.
+__device__ __forceinline__ double flux_j1_arg(double b, double q) { return b * q * (1.0 / kHc); }
+
+// The integrand in two steps: the spline record's load (issued early) and
+// the rest once J1 is known.
+struct FluxNode {
+  double q, t, tc;
+  int i;
+  double4 c;  // spline interval record (in flight after flux_prelude)
+};
+
+// load == false leaves the record zero and reads nothing (a caller's idle
+// lane, whose table pointer need not be valid).
+__device__ __forceinline__ void flux_prelude(const double4* __restrict__ ff, double w2g2, double q, FluxNode* n,
+                                             bool load = true)
+{
+  n->q = q;
+  n->t = q * q + w2g2;
+  n->tc = n->t < kQ2max ? n->t : kQ2max - kDQ2;
+  int i = (int)((n->tc - kQ2min) * (1.0 / kDQ2));
+  n->i = i < 0 ? 0 : (i > kNQ2 - 2 ? kNQ2 - 2 : i);
+  n->c = load ? ff[n->i] : make_double4(0.0, 0.0, 0.0, 0.0);
+}
+
+__device__ __forceinline__ double flux_finish(const FluxNode& n, double j1)
+{
+  const double d = n.tc - knot(kQ2min, kDQ2, n.i);
+  const double f = n.c.x + d * (n.c.y + d * (n.c.z + d * n.c.w));
+  const double v = n.q * n.q * f * rcp_nr(n.t) * j1;  // t >= (k / g1)^2 > 0
+  return n.tc > kQ2Last ? __builtin_nan("") : v;
+}
+
+template <int BR>
+__device__ __forceinline__ double bessel_j1_branch(double x)
+{
+  return BR == 1 ? bessel_j1_small(x) : (BR == 2 ? bessel_j1_large(x) : bessel_j1(x));
+}
+
+template <int BR>
+__device__ __forceinline__ double flux_integrand(const double4* __restrict__ ff, double b, double w2g2, double q)
+{
+  FluxNode n;
+  flux_prelude(ff, w2g2, q, &n);
+  return flux_finish(n, bessel_j1_branch<BR>(flux_j1_arg(b, q)));
+}
+
 struct FluxFormIntegrand {
   const double4* __restrict__ ff;
   double b, w2g2;
-  // The spline record is a dependent load from a 32 MB table: it is issued
-  // first and J1 (independent ALU work) runs while it is in flight.  The
-  // interval is the rounded estimate alone, as in spline_eval_near: within
-  // rounding of a knot the neighbouring cubic agrees to a few ulp.
-  __device__ __forceinline__ double operator()(double q) const
-  {
-    const double t = q * q + w2g2;
-    const double tc = t < kQ2max ? t : kQ2max - kDQ2;
-    int i = (int)((tc - kQ2min) * (1.0 / kDQ2));
-    i = i < 0 ? 0 : (i > kNQ2 - 2 ? kNQ2 - 2 : i);
-    const double4 c = ff[i];
-    const double j1 = bessel_j1(b * q * (1.0 / kHc));
-    const double d = tc - knot(kQ2min, kDQ2, i);
-    const double f = c.x + d * (c.y + d * (c.z + d * c.w));
-    const double v = q * q * f * rcp_nr(t) * j1;  // t >= (k / g1)^2 > 0
-    return tc > kQ2Last ? __builtin_nan("") : v;
-  }
+  __device__ __forceinline__ double operator()(double q) const { return flux_integrand<0>(ff, b, w2g2, q); }
 };
 
 // One form-factor QAGS task of a flux row: its node (b, k) and the flux slot
@@ -216,6 +254,7 @@ struct FluxTask {
 
 // Task sources of the QAGS kernel: get(t) describes task t.
 struct GridTask {  // flux-row nodes of one context
+  static constexpr bool kSharedFF = true;  // one form-factor table for every task (ff)
   const TaskRec* __restrict__ recs;
   FluxParams fp;
   const double4* ff;
@@ -236,6 +275,7 @@ struct GridTask {  // flux-row nodes of one context
 };
 
 struct PointTask {  // arbitrary (b, k) points of one context
+  static constexpr bool kSharedFF = true;
   const double* __restrict__ b;
   const double* __restrict__ k;
   const int* __restrict__ idx;  // compacted indices of points needing QAGS
@@ -257,63 +297,224 @@ struct PointTask {  // arbitrary (b, k) points of one context
 };
 
 // ---------------------------------------------------------------------------
-// Form-factor flux: persistent QAGS kernel.  Each lane owns one integral at a
-// time; lanes take new tasks from a global cursor as they finish, so the
-// wavefront keeps issuing full-width QK21 passes.  stats: [0] integrand
-// evaluations, [1] integrals with ier != 0, [2] overflow count.
+// Form-factor flux: persistent QAGS kernel.
+//
+// Each lane owns one integral at a time (the dqagse state machine of
+// upc_qags.h, its interval lists in private memory) and takes a new task
+// from a global cursor when it finishes.  A loop iteration is one dqagse
+// step for every lane: the first QK21 pass of a new task, or the two halves
+// of a bisection, in at most two rounds of one pass per lane.
+//
+// A round evaluates the 21 nodes of every lane's pass as one pool: each lane
+// writes its abscissae to its row of the wavefront's LDS table and queues
+// its nodes as J1 small-argument items (x = b q / hc <= 8, front of the
+// queue) or large-argument items (after them).  The wavefront then sweeps
+// the queue 64 items at a time, J1's small branch over the front and its
+// large branch over the rest, each lane evaluating whichever node the queue
+// hands it (the owner's b and (k/g)^2 come over ds_bpermute) and writing
+// the value over the abscissa.  Every sweep runs one J1 branch on all 64
+// lanes, where evaluating each lane's own nodes in turn would run both
+// branches on almost every node (the two kinds are about evenly mixed in a
+// grid: 47% small, 53% large on the 1001 x 121 Pb-Pb grid).  Finally each
+// lane sums its own row in QK21's order (qk21_stored), so results are the
+// same bits as a lane evaluating its own nodes.
+//
+// The LDS table (13,440 B per wavefront) sets the occupancy: 3 workgroups
+// of 4 wavefronts per CU, 168 VGPRs per lane.
+//
+// stats: [0] integrand evaluations, [1] integrals with ier != 0, [2]
+// overflow count.
 // ---------------------------------------------------------------------------
-// Waves per SIMD the QAGS kernel is compiled for (its register budget).
-#ifndef UPC_QAGS_MIN_WAVES
-#define UPC_QAGS_MIN_WAVES 4
-#endif
+struct QagsWaveLds {
+  double v[64 * 21];             // [owner lane][node e]: abscissa, then the integrand value
+  unsigned short item[21 * 64];  // owner lane << 5 | e: small-argument items first, large after
+};
+constexpr int kQagsBlock = 256;      // threads of a workgroup (4 wavefronts, 53,760 B of LDS)
+constexpr int kQagsBlocksPerCu = 3;  // resident workgroups per CU (LDS-bound)
 
-// First QK21 pass of every task (the whole range [0, 10]): one task per lane
-// at a time, every lane busy, results kept for k_flux_qags.
-template <class TS>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(UPC_QAGS_MIN_WAVES, 8))) void k_flux_first(
-    TS ts, const unsigned long long* __restrict__ ntasks_p, double4* __restrict__ first)
+__device__ __forceinline__ void wave_lds_sync()
 {
-  __shared__ double fvbuf[20 * 256];
-  const long ntasks = (long)*ntasks_p;
-  double* fv = fvbuf + (threadIdx.x >> 6) * 20 * 64 + (threadIdx.x & 63);
-  const long stride = (long)gridDim.x * blockDim.x;
-  FluxFormIntegrand f;
-  for (long t0 = (long)blockIdx.x * blockDim.x; t0 < ntasks; t0 += stride) {
-    const long t = t0 + (threadIdx.x & ~63) + (threadIdx.x & 63);
-    const bool live = t < ntasks;
-    FluxTask ft{};
-    ts.get(live ? t : ntasks - 1, &ft);
-    f.ff = ft.ff;
-    f.b = ft.b;
-    f.w2g2 = ft.k * ft.k / ft.g1 / ft.g1;
-    const QK21Out o = qk21_lds(f, kQagsA, kQagsB, fv, 64);
-    if (live) first[t] = make_double4(o.result, o.abserr, o.resabs, o.resasc);
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+}
+
+__device__ __forceinline__ int lanes_below(unsigned long long m)
+{
+  return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+}
+
+// Items [beg, end) of the queue with J1 branch BR, 64 per sweep.  beg, end
+// are wavefront-uniform, so every lane runs every ds_bpermute; a lane past
+// end takes a dummy item (abscissa 0) and reads no table, since its own
+// table pointer is null when it never held a task, and stores nothing.
+struct SweepItem {
+  bool act;
+  int slot;      // owner * 21 + e
+  double x;      // J1 argument
+  FluxNode n;
+};
+
+template <bool SHARED_FF>
+__device__ __forceinline__ void sweep_fetch(QagsWaveLds& W, int lane, int r0, int end, const double4* ff, double b,
+                                            double w2g2, SweepItem* it)
+{
+  const int r = r0 + lane;
+  it->act = r < end;
+  const int code = it->act ? (int)W.item[r] : lane << 5;
+  const int owner = code >> 5, e = code & 31;
+  const double bo = __shfl(b, owner), wo = __shfl(w2g2, owner);
+  const double4* ffo = ff;
+  if (!SHARED_FF) ffo = (const double4*)__shfl((long long)ff, owner);
+  it->slot = owner * 21 + e;
+  const double q = it->act ? W.v[it->slot] : 0.0;
+  it->x = flux_j1_arg(bo, q);
+  flux_prelude(ffo, wo, q, &it->n, it->act);
+}
+
+template <int BR>
+__device__ __forceinline__ void sweep_finish(QagsWaveLds& W, const SweepItem& it)
+{
+  const double v = flux_finish(it.n, bessel_j1_branch<BR>(it.x));
+  if (it.act) W.v[it.slot] = v;
+}
+
+// Two items per lane in flight: both fetches (queue entry, owner's
+// parameters, abscissa, spline record load) are issued before either J1.
+template <int BR, bool SHARED_FF>
+__device__ __forceinline__ void qags_sweeps(QagsWaveLds& W, int lane, int beg, int end, const double4* ff,
+                                            double b, double w2g2)
+{
+#pragma unroll 1
+  for (int r0 = beg; r0 < end; r0 += 128) {
+    SweepItem ia, ib;
+    sweep_fetch<SHARED_FF>(W, lane, r0, end, ff, b, w2g2, &ia);
+    sweep_fetch<SHARED_FF>(W, lane, r0 + 64, end, ff, b, w2g2, &ib);
+    sweep_finish<BR>(W, ia);
+    sweep_finish<BR>(W, ib);
   }
 }
 
-// first (may be null): the tasks' first QK21 passes from k_flux_first.  With
-// it, a lane that takes a task applies that pass at once (finishing the
-// integral when it already meets the tolerance), so every step in the loop
-// below is a two-pass bisection step.
-template <int CAP, class TS>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(UPC_QAGS_MIN_WAVES, 8))) void k_flux_qags(
-    TS ts, const unsigned long long* __restrict__ ntasks_p, unsigned long long* __restrict__ cursor,
-    double* __restrict__ flux, int* __restrict__ ier_out, unsigned long long* __restrict__ stats,
-    long long* __restrict__ overflow, int fast_cap, const double4* __restrict__ first)
+// Node of QK21 in ascending abscissa order k = 0..20: the left nodes
+// centr - hlgth kXgk[k] (kXgk decreasing), the centre, the right nodes.
+__device__ __forceinline__ int qk21_qorder_node(int k)
 {
-  __shared__ double fvbuf[20 * 256];
+  return k < 10 ? qk21_node(k, 0) : (k == 10 ? 0 : qk21_node(20 - k, 1));
+}
+
+// Exclusive prefix sum over the wavefront's lanes of x in [0, 32), and the
+// total: one ballot per bit.
+__device__ __forceinline__ int wave_prefix5(int x, int* total)
+{
+  int pre = 0, tot = 0;
+#pragma unroll
+  for (int bit = 0; bit < 5; ++bit) {
+    const unsigned long long m = __ballot((x >> bit) & 1);
+    pre += lanes_below(m) << bit;
+    tot += __popcll(m) << bit;
+  }
+  *total = tot;
+  return pre;
+}
+
+// One QK21 pass per lane with mine set, on [qa, qb]; the result for those
+// lanes (zeros elsewhere).  ff, b, w2g2: the lane's own integrand.
+//
+// J1's argument b q / hc grows with the abscissa, so a lane's small-argument
+// nodes are the first ns of its nodes in abscissa order: the lane's items
+// are two runs, [ps, ps + ns) among the small items and the rest among the
+// large ones, placed by one prefix sum over the lanes.
+template <bool SHARED_FF>
+__device__ __forceinline__ QK21Out qk21_wave(QagsWaveLds& W, int lane, bool mine, double qa, double qb,
+                                             const double4* ff, double b, double w2g2)
+{
+  const double centr = 0.5 * (qa + qb), hlgth = 0.5 * (qb - qa);
+  double* col = W.v + lane * 21;
+  wave_lds_sync();
+  int ns = 0;
+#pragma unroll 1
+  for (int e = 0; e < 21; ++e) {
+    const double q = qk21_abscissa(e, centr, hlgth);
+    col[e] = q;
+    ns += flux_j1_arg(b, q) <= 8.0;
+  }
+  if (!mine) ns = 0;
+  int S = 0;
+  const int ps = wave_prefix5(ns, &S);
+  const unsigned long long act = __ballot(mine);
+  const int ntot = 21 * __popcll(act);
+  const int pl = S + 21 * lanes_below(act) - ps;  // start of the lane's large run
+  if (mine) {
+#pragma unroll 1
+    for (int k = 0; k < 21; ++k) {
+      const int e = qk21_qorder_node(k);
+      W.item[k < ns ? ps + k : pl + (k - ns)] = (unsigned short)(lane << 5 | e);
+    }
+  }
+  wave_lds_sync();
+  qags_sweeps<1, SHARED_FF>(W, lane, 0, S, ff, b, w2g2);
+  qags_sweeps<2, SHARED_FF>(W, lane, S, ntot, ff, b, w2g2);
+  wave_lds_sync();
+  QK21Out o{};
+  if (mine) o = qk21_stored(col, 1, qa, qb);
+  return o;
+}
+
+template <int CAP, class TS>
+__global__ __launch_bounds__(kQagsBlock) __attribute__((amdgpu_waves_per_eu(kQagsBlocksPerCu, kQagsBlocksPerCu)))
+void k_flux_qags(TS ts, const unsigned long long* __restrict__ ntasks_p, unsigned long long* __restrict__ cursor,
+                 double* __restrict__ flux, int* __restrict__ ier_out, unsigned long long* __restrict__ stats,
+                 long long* __restrict__ overflow, int fast_cap)
+{
+  __shared__ QagsWaveLds lds[kQagsBlock / 64];
+  QagsWaveLds& W = lds[threadIdx.x >> 6];
   typename QagsSmallStore<CAP>::Arrays lists;
   Qags<QagsSmallStore<CAP>> q;
   q.s.a = &lists;
   q.s.lim = fast_cap < CAP ? fast_cap : CAP;
-  const long ntasks = (long)*ntasks_p;
+  const long long ntasks = (long long)*ntasks_p;
   const int lane = threadIdx.x & 63;
-  double* fv = fvbuf + (threadIdx.x >> 6) * 20 * 64 + lane;
-  long task = -1;  // -1: needs a task; LONG_MAX: exhausted
+  long long task = -1;  // -1: needs a task; LLONG_MAX: exhausted
   FluxTask ft{};
-  FluxFormIntegrand f;
+  double w2g2 = 0.0;
   unsigned long long my_evals = 0, my_bad = 0;
-  auto retire = [&](int st) {  // a finished or overflowing integral leaves the lane
+  while (true) {
+    bool need = task < 0;
+    unsigned long long mask;
+    while ((mask = __ballot(need)) != 0) {
+      const int leader = __ffsll((long long)mask) - 1;
+      unsigned long long base = 0;
+      if (lane == leader) base = atomicAdd(cursor, (unsigned long long)__popcll(mask));
+      base = __shfl(base, leader);
+      if (need) {
+        const long long t = (long long)(base + (unsigned long long)lanes_below(mask));
+        if (t < ntasks) {
+          task = t;
+          ts.get(t, &ft);
+          w2g2 = ft.k * ft.k / ft.g1 / ft.g1;
+          q.start();
+        } else {
+          task = LLONG_MAX;
+        }
+        need = false;
+      }
+    }
+    if (__ballot(task != LLONG_MAX) == 0) break;
+    const bool live = task != LLONG_MAX;
+    const int np = live ? q.passes() : 0;
+    const double4* ff;
+    if constexpr (TS::kSharedFF) ff = ts.ff;
+    else ff = ft.ff;
+    double qa = 0.0, qb = 0.0;
+    if (live) q.interval(0, &qa, &qb);
+    const QK21Out o = qk21_wave<TS::kSharedFF>(W, lane, live, qa, qb, ff, ft.b, w2g2);
+    QK21Out o2{};
+    if (__ballot(np == 2) != 0) {
+      if (np == 2) q.interval(1, &qa, &qb);
+      o2 = qk21_wave<TS::kSharedFF>(W, lane, np == 2, qa, qb, ff, ft.b, w2g2);
+    }
+    if (!live) continue;
+    const int st = q.consume(o, o2);
     if (st == kQagsDone) {
       const double Q = q.result / ft.A;
       flux[ft.out] = ft.factor * Q * Q / ft.k;
@@ -327,53 +528,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(UPC_QAGS_MI
       const unsigned long long slot = atomicAdd(&stats[2], 1ull);
       overflow[slot] = task;
       task = -1;
-    }
-  };
-  while (true) {
-    bool need = task < 0;
-    unsigned long long mask;
-    while ((mask = __ballot(need)) != 0) {
-      const int leader = __ffsll((long long)mask) - 1;
-      unsigned long long base = 0;
-      if (lane == leader) base = atomicAdd(cursor, (unsigned long long)__popcll(mask));
-      base = __shfl(base, leader);
-      if (need) {
-        const unsigned long long rank = __popcll(mask & ((1ull << lane) - 1ull));
-        const long t = (long)(base + rank);
-        if (t < ntasks) {
-          task = t;
-          ts.get(t, &ft);
-          f.ff = ft.ff;
-          f.b = ft.b;
-          f.w2g2 = ft.k * ft.k / ft.g1 / ft.g1;
-          q.start();
-          need = false;
-          if (first) {
-            const double4 v = first[t];
-            const QK21Out o{v.x, v.y, v.z, v.w};
-            const int st = q.consume(o, o);
-            retire(st);
-            need = task < 0;
-          }
-        } else {
-          task = LONG_MAX;
-          need = false;
-        }
-      }
-    }
-    if (__ballot(task != LONG_MAX) == 0) break;
-    if (task != LONG_MAX) {
-      QK21Out o, o2{};
-      const int np = q.passes();
-#pragma unroll 1
-      for (int k = 0; k < np; ++k) {  // one copy of the integrand for both passes
-        double qa, qb;
-        q.interval(k, &qa, &qb);
-        const QK21Out r = qk21_lds(f, qa, qb, fv, 64);
-        if (k == 0) o = r;
-        else o2 = r;
-      }
-      retire(q.consume(o, o2));
     }
   }
   for (int o = 32; o > 0; o >>= 1) {
@@ -392,7 +546,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(UPC_QAGS_MI
 // rerun from the start here, one wavefront per integral, with the reference's
 // 1000-interval workspace in LDS.  Lanes 0..41 evaluate the 42 nodes of a
 // step (both halves) at once; lane 0 sums each pass in QK21's order (the same
-// qk21_sum as the fast kernel) and runs the dqagse state machine, so the
+// qk21_core as the fast kernel) and runs the dqagse state machine, so the
 // serial part of a step is LDS work and one sum per pass.
 constexpr int kQagsWaveBlocks = 1024;  // workgroups of the overflow kernel (36.5 KB LDS each)
 
@@ -403,7 +557,6 @@ __global__ __launch_bounds__(64) void k_flux_qags_wave(TS ts, const long long* _
 {
   __shared__ QagsBigArrays ws;
   __shared__ double vals[2][21];
-  __shared__ double fvs[20];
   __shared__ double iv[2][2];
   __shared__ int ctl[2];  // [0] passes of the step, [1] state after it
   const long n = (long)stats[2];
@@ -434,9 +587,9 @@ __global__ __launch_bounds__(64) void k_flux_qags_wave(TS ts, const long long* _
       }
       __syncthreads();
       if (lane == 0) {
-        const QK21Out o = qk21_sum([&](int e, double) { return vals[0][e]; }, iv[0][0], iv[0][1], fvs, 1);
+        const QK21Out o = qk21_stored(vals[0], 1, iv[0][0], iv[0][1]);
         QK21Out o2{};
-        if (np == 2) o2 = qk21_sum([&](int e, double) { return vals[1][e]; }, iv[1][0], iv[1][1], fvs, 1);
+        if (np == 2) o2 = qk21_stored(vals[1], 1, iv[1][0], iv[1][1]);
         ctl[1] = q.consume(o, o2);
       }
       __syncthreads();

@@ -70,7 +70,7 @@ static int qags_persistent_blocks()
   (void)hipGetDevice(&dev);
   (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
   if (ncu <= 0) ncu = 256;
-  return ncu * UPC_QAGS_MIN_WAVES;  // 256-lane workgroups: one wave per SIMD each
+  return ncu * kQagsBlocksPerCu;  // as many workgroups as stay resident
 }
 
 // QAGS nodes of one flux row are at most 97: node j has b_j >= bmin e^{j logd}
@@ -81,7 +81,7 @@ constexpr int kMaxQagsNodesPerRow = 98;
 // Work buffers for npts points and their flux rows (at most 2 npts); task
 // records, offsets and overflow lists are 64-bit.  The point-charge flux has
 // no QAGS tasks, so its task buffers stay minimal; the per-point form-factor
-// call needs `tasks` of them (its point indices and first passes).
+// call needs `tasks` of them (its point indices).
 static int ensure_work(upc_ctx* c, size_t npts, size_t rows, size_t tasks = 0)
 {
   LumiWork& w = c->work;
@@ -98,7 +98,6 @@ static int ensure_work(upc_ctx* c, size_t npts, size_t rows, size_t tasks = 0)
   UPC_HIP(hipMalloc(&w.d_row_off, sizeof(long long) * rows));
   UPC_HIP(hipMalloc(&w.d_flux, sizeof(double) * rows * kNB1));
   UPC_HIP(hipMalloc(&w.d_tasks, sizeof(TaskRec) * ntask));
-  UPC_HIP(hipMalloc(&w.d_first, sizeof(double4) * ntask));
   UPC_HIP(hipMalloc(&w.d_overflow, sizeof(long long) * ntask));
   UPC_HIP(hipMalloc(&w.d_counters, sizeof(unsigned long long) * 8));
   UPC_HIP(hipMalloc(&w.d_points, sizeof(double) * 4 * npts));
@@ -112,8 +111,8 @@ static int ensure_work(upc_ctx* c, size_t npts, size_t rows, size_t tasks = 0)
 void free_work(upc_ctx* c)
 {
   LumiWork& w = c->work;
-  void* ptrs[] = {w.d_row_k,    w.d_row_logd, w.d_row_cnt, w.d_row_off, w.d_flux, w.d_tasks,
-                  w.d_first,    w.d_overflow, w.d_counters, w.d_points, w.d_part};
+  void* ptrs[] = {w.d_row_k, w.d_row_logd, w.d_row_cnt, w.d_row_off, w.d_flux,
+                  w.d_tasks, w.d_overflow, w.d_counters, w.d_points, w.d_part};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   w = LumiWork{};
@@ -152,11 +151,9 @@ static int run_flux(upc_ctx* c, int npts, int mirror_ny, hipStream_t s)
   if (!c->cfg.flux_point) {
     PhaseScope ph(c, s, UPC_PHASE_QAGS);
     GridTask ts{(const TaskRec*)w.d_tasks, fp, c->d_ff};
-    hipLaunchKernelGGL((k_flux_first<GridTask>), dim3(qags_persistent_blocks()), dim3(256), 0, s, ts,
-                       w.d_counters + 4, w.d_first);
-    hipLaunchKernelGGL((k_flux_qags<kQagsCap, GridTask>), dim3(qags_persistent_blocks()), dim3(256), 0, s, ts,
-                       w.d_counters + 4, w.d_counters + 5, w.d_flux, (int*)nullptr, w.d_counters,
-                       w.d_overflow, c->qags_cap, (const double4*)w.d_first);
+    hipLaunchKernelGGL((k_flux_qags<kQagsCap, GridTask>), dim3(qags_persistent_blocks()), dim3(kQagsBlock), 0, s,
+                       ts, w.d_counters + 4, w.d_counters + 5, w.d_flux, (int*)nullptr, w.d_counters,
+                       w.d_overflow, c->qags_cap);
     hipLaunchKernelGGL((k_flux_qags_wave<GridTask>), dim3(kQagsWaveBlocks), dim3(64), 0, s, ts, w.d_overflow,
                        w.d_counters, w.d_flux, (int*)nullptr);
     UPC_HIP(hipGetLastError());
@@ -381,9 +378,8 @@ __global__ void k_zero_u64(unsigned long long* __restrict__ p, int n)
 }
 
 // The per-point form-factor flux on the buffers in w: classify (point flux
-// beyond 2R, QAGS tasks inside), first QK21 passes, the persistent QAGS
-// kernel, then the 1000-interval kernel for the tasks that outgrew the fast
-// list.  Counters are cleared by a kernel, so the sequence may be captured.
+// beyond 2R, QAGS tasks inside), the persistent QAGS kernel, then the
+// 1000-interval kernel for the tasks that outgrew the fast list.  Counters are cleared by a kernel, so the sequence may be captured.
 int enqueue_flux_form(upc_ctx* c, const double* d_b, const double* d_k, int n, double* d_out, int* d_ier,
                       const FluxFormBufs& w, hipStream_t s)
 {
@@ -392,11 +388,8 @@ int enqueue_flux_form(upc_ctx* c, const double* d_b, const double* d_k, int n, d
   hipLaunchKernelGGL(k_flux_form_classify, dim3(nblk(n, 256)), dim3(256), 0, s, d_b, d_k, n, fp, d_out, d_ier,
                      w.idx, w.counters);
   PointTask ts{d_b, d_k, w.idx, fp, c->d_ff};
-  hipLaunchKernelGGL((k_flux_first<PointTask>), dim3(qags_persistent_blocks()), dim3(256), 0, s, ts,
-                     w.counters + 4, w.first);
-  hipLaunchKernelGGL((k_flux_qags<kQagsCap, PointTask>), dim3(qags_persistent_blocks()), dim3(256), 0, s, ts,
-                     w.counters + 4, w.counters + 5, d_out, d_ier, w.counters, w.overflow, c->qags_cap,
-                     (const double4*)w.first);
+  hipLaunchKernelGGL((k_flux_qags<kQagsCap, PointTask>), dim3(qags_persistent_blocks()), dim3(kQagsBlock), 0, s,
+                     ts, w.counters + 4, w.counters + 5, d_out, d_ier, w.counters, w.overflow, c->qags_cap);
   hipLaunchKernelGGL((k_flux_qags_wave<PointTask>), dim3(kQagsWaveBlocks), dim3(64), 0, s, ts, w.overflow,
                      w.counters, d_out, d_ier);
   UPC_HIP(hipGetLastError());
@@ -411,7 +404,7 @@ int launch_flux_form(upc_ctx* c, const double* d_b, const double* d_k, int n, do
   int rc = ensure_work(c, 1, 1, (size_t)n);
   if (rc) return rc;
   LumiWork& w = c->work;
-  const FluxFormBufs bufs{(int*)w.d_tasks, w.d_first, w.d_overflow, w.d_counters};
+  const FluxFormBufs bufs{(int*)w.d_tasks, w.d_overflow, w.d_counters};
   rc = enqueue_flux_form(c, d_b, d_k, n, d_out, d_ier, bufs, s);
   if (rc) return rc;
   return read_stats(c, s);

@@ -41,7 +41,6 @@ struct LumiWork {
   double* d_flux = nullptr;       // [rows][120]
   size_t tasks_cap = 0;
   void* d_tasks = nullptr;        // QAGS task records (b, k, flux slot); per-point calls: point indices
-  double4* d_first = nullptr;     // first QK21 pass of each task (result, abserr, resabs, resasc)
   unsigned long long* d_counters = nullptr;  // [0] task cursor, [1] qags bad, [2] evals, [3] overflow n
   long long* d_overflow = nullptr;  // tasks that exceeded the fast workspace
   double* d_points = nullptr;     // (M, Y, dm, dy) per point
@@ -52,7 +51,6 @@ struct LumiWork {
 // Buffers of one per-point form-factor flux launch (upc_flux_form).
 struct FluxFormBufs {
   int* idx;                      // [n] compacted indices of the points that need QAGS
-  double4* first;                // [n] their first QK21 passes
   long long* overflow;           // [n] tasks that outgrew the fast interval list
   unsigned long long* counters;  // [8] evaluations, failed, overflow count, -, tasks, cursor
 };

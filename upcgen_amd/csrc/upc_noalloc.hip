@@ -21,8 +21,7 @@ constexpr int kMaxPoints = 1 << 18;  // points per call (262,144)
 // Layout of the scratch array (offsets in bytes, each 256-aligned).
 constexpr size_t align256(size_t x) { return (x + 255) / 256 * 256; }
 constexpr size_t kOffIdx = 0;
-constexpr size_t kOffFirst = align256(kOffIdx + sizeof(int) * kMaxPoints);
-constexpr size_t kOffOverflow = align256(kOffFirst + sizeof(double4) * kMaxPoints);
+constexpr size_t kOffOverflow = align256(kOffIdx + sizeof(int) * kMaxPoints);
 constexpr size_t kOffCounters = align256(kOffOverflow + sizeof(long long) * kMaxPoints);
 constexpr size_t kScratchBytes = align256(kOffCounters + sizeof(unsigned long long) * 8);
 
@@ -75,7 +74,6 @@ extern "C" int upc_flux_form_noalloc(upc_ctx* c, const double* d_b, const double
   unsigned char* base = nullptr;
   if (int rc = scratch_ptr(dev, &base)) return rc;
   const FluxFormBufs w{(int*)(base + kOffIdx),
-                       (double4*)(base + kOffFirst),
                        (long long*)(base + kOffOverflow),
                        (unsigned long long*)(base + kOffCounters)};
   return enqueue_flux_form(c, d_b, d_k, n, d_out, d_ier, w, (hipStream_t)stream);

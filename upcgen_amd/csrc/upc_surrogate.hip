@@ -710,6 +710,8 @@ int upc_sur_train(upc_sur* s, const float* d_X, const float* d_T, long long n, i
     float* m = s->d_m + (long)net * kSurParams;
     float* v = s->d_v + (long)net * kSurParams;
     hipLaunchKernelGGL(k_batch_zero_f, dim3(nblk(steps + 1, 256)), dim3(256), 0, st, s->d_loss, steps + 1);
+    hipLaunchKernelGGL(k_batch_zero_f, dim3(nblk(kSurParams, 256)), dim3(256), 0, st, m, kSurParams);
+    hipLaunchKernelGGL(k_batch_zero_f, dim3(nblk(kSurParams, 256)), dim3(256), 0, st, v, kSurParams);
     hipLaunchKernelGGL(k_batch_zero_i, dim3(1), dim3(1), 0, st, s->d_step);
     const unsigned long long seed = s->seed * 1000003ull + 17ull * net;
     auto step_fn = [&](hipStream_t q) -> int {

@@ -52,6 +52,7 @@ class Surrogate:
 
     def __init__(self, seed: int = 1):
         _dev()
+        self.seed = seed
         self._s = C.c_void_p()
         _check(load().upc_sur_create(seed, C.byref(self._s)), "upc_sur_create")
         self.nparams = param_count()
@@ -166,6 +167,40 @@ class Surrogate:
         ra, rb = mu + sig * pa.double(), mu + sig * pb.double()
         return logL0 + 0.5 * (ra + rb), (ra - rb).abs()
 
+    def copy(self, seed=None):
+        """A new surrogate with the same weights, normalisation and error scale."""
+        c = Surrogate(self.seed if seed is None else seed)
+        for net in (0, 1):
+            c.set_params(net, self.get_params(net))
+        c.norm = self.norm
+        c.err_scale = self.err_scale
+        return c
+
+    def fit_system(self, config, n=16384, n_valid=4096, steps=4000, batch=4096, lr=3e-4, max_err=1e-3,
+                   coverage=0.99, seed=7):
+        """A copy of this surrogate fine-tuned on one system: n FP64
+        quadrature labels of `config` at uniform (M, Y) over its grid range
+        (inside the domain), `steps` Adam steps from the current weights, and
+        the error scale calibrated on n_valid further points of the system.
+        Returns (surrogate, seconds spent on labels, seconds of training)."""
+        import time
+        torch = _dev()
+        cfg = Config.make(**{f: getattr(config, f) for f, _ in Config._fields_})
+        mr, yr = (cfg.mmin, cfg.mmax), (cfg.ymin, cfg.ymax)
+        t0 = time.perf_counter()
+        X, r, _, _ = make_dataset(0, n, seed, systems=[cfg], mrange=mr, yrange=yr, log_m=False)
+        Xv, _, lgv, L0v = make_dataset(0, n_valid, seed + 101, systems=[cfg], mrange=mr, yrange=yr, log_m=False)
+        torch.cuda.synchronize()
+        t_label = time.perf_counter() - t0
+        s = self.copy()
+        mu, sig = s.norm
+        t0 = time.perf_counter()
+        s.train(X.contiguous(), ((r - mu) / sig).float().contiguous(), steps=steps, batch=batch, lr=lr)
+        torch.cuda.synchronize()
+        t_train = time.perf_counter() - t0
+        s.calibrate(Xv.contiguous(), L0v, lgv, max_err=max_err, coverage=coverage)
+        return s, t_label, t_train
+
     def attach(self, ctx: Context, max_err=1e-3):
         _check(load().upc_ctx_set_surrogate(ctx._ctx, self._s, max_err), "upc_ctx_set_surrogate")
         ctx._surrogate = self  # keep alive while attached
@@ -199,17 +234,47 @@ def sample_systems(n, seed):
     return out
 
 
-def make_dataset(n_systems, per_system, seed, systems=None):
+XI_MAX = 4.0  # domain of the network: both photons with xi = k R / (gamma hc) <= XI_MAX
+M_PROTON = 0.9382720813
+
+
+def xi_max(cfg, M, Y):
+    """max(xi1, xi2), xi = k R / (gamma hc), k = M/2 e^{+-Y}: the domain test."""
+    gamma = cfg.sqrts / (2 * M_PROTON)
+    return np.asarray(M) / 2 * np.exp(np.abs(np.asarray(Y))) * cfg.R / (gamma * 0.1973269718)
+
+
+def sample_points(cfg, n, rng, mrange=(1.0, 200.0), yrange=(-7.0, 7.0), log_m=True):
+    """n points (M, Y) of the training distribution (M log-uniform or uniform
+    in mrange, Y uniform in yrange) inside the network's domain: draws
+    outside it are rejected and drawn again.  Fewer than n come back only
+    when the domain holds almost none of the range (at most 64 rounds)."""
+    Ms, Ys, got = [], [], 0
+    for _ in range(64):
+        k = max(2 * (n - got), 1024)
+        M = np.exp(rng.uniform(np.log(mrange[0]), np.log(mrange[1]), k)) if log_m else rng.uniform(*mrange, k)
+        Y = rng.uniform(yrange[0], yrange[1], k)
+        keep = xi_max(cfg, M, Y) <= XI_MAX
+        Ms.append(M[keep][:n - got])
+        Ys.append(Y[keep][:n - got])
+        got += len(Ms[-1])
+        if got >= n:
+            break
+    return np.concatenate(Ms), np.concatenate(Ys)
+
+
+def make_dataset(n_systems, per_system, seed, systems=None, mrange=(1.0, 200.0), yrange=(-7.0, 7.0),
+                 log_m=True):
     """Features, targets r = log L - log L0 and the labels, from the FP64 GPU
-    quadrature (upc_lumi_points) at random (M, Y) of random systems.
-    Points where L or L0 is zero or not finite are dropped."""
+    quadrature (upc_lumi_points) at per_system random (M, Y) inside the
+    domain (sample_points) of each of n_systems random systems (or of
+    `systems`).  Points where L is zero or not finite are dropped."""
     torch = _dev()
     rng = np.random.default_rng(seed + 1)
     systems = systems if systems is not None else sample_systems(n_systems, seed)
     Xs, Rs, Ls, L0s = [], [], [], []
     for cfg in systems:
-        M = np.exp(rng.uniform(0.0, np.log(200.0), per_system))
-        Y = rng.uniform(-7.0, 7.0, per_system)
+        M, Y = sample_points(cfg, per_system, rng, mrange, yrange, log_m)
         ctx = Context(cfg)
         L = ctx.lumi_points(torch.as_tensor(M, device="cuda"), torch.as_tensor(Y, device="cuda"))
         ctx.close()
@@ -223,10 +288,10 @@ def make_dataset(n_systems, per_system, seed, systems=None):
     return torch.cat(Xs), torch.cat(Rs), torch.cat(Ls), torch.cat(L0s)
 
 
-def train_default(n_systems=256, per_system=4096, steps=6000, batch=4096, lr=2e-3, seed=11, n_valid=32):
-    """The default training run (n_systems x per_system points, 2^20 by
-    default) with the error scale calibrated on n_valid further systems.
-    Returns (surrogate, loss curves, training rows)."""
+def train_default(n_systems=256, per_system=4096, steps=30000, batch=8192, lr=2e-3, seed=11, n_valid=32):
+    """The default training run (n_systems x per_system points inside the
+    domain, 2^20 by default) with the error scale calibrated on n_valid
+    further systems.  Returns (surrogate, loss curves, training rows)."""
     X, r, _, _ = make_dataset(n_systems, per_system, seed)
     mu, sig = float(r.mean()), float(r.std())
     T = ((r - mu) / sig).float().contiguous()
