@@ -306,6 +306,12 @@ int upc_sur_get_params(upc_sur *sur, int net, float *h_out);
 int upc_sur_set_params(upc_sur *sur, int net, const float *h_in);
 int upc_sur_set_norm(upc_sur *sur, double mu, double sigma);
 int upc_sur_get_norm(upc_sur *sur, double *h_mu, double *h_sigma);
+/* Product precision of both networks (forward, gradients, inference):
+ * split = 1 (default) splits every operand into bf16 hi + lo and sums
+ * hi*hi + hi*lo + lo*hi on MFMA with FP32 accumulation (about FP32 products),
+ * the output layer uses the FP32 weights; split = 0 rounds each operand to
+ * one bf16 (and the output weights too). */
+int upc_sur_set_precision(upc_sur *sur, int split);
 /* Predicted error of a bin = scale * |r_a - r_b| (the ensemble spread scaled
  * by a factor calibrated on held-out data; default 1). */
 int upc_sur_set_err_scale(upc_sur *sur, double scale);
@@ -332,6 +338,15 @@ int upc_sur_train(upc_sur *sur, const float *d_X, const float *d_T, long long n,
  * recomputed by the FP64 quadrature (the spread times the surrogate's error
  * scale), as are bins outside its domain.  sur = NULL detaches. */
 int upc_ctx_set_surrogate(upc_ctx *ctx, upc_sur *sur, double max_err);
+/* Error check of the surrogate grid form (no reference counterpart).  With
+ * anchor spacings every_m, every_y > 0 (default 8 and 4) the FP64 quadrature
+ * is run at the anchor lattice (every every_m-th M bin x every every_y-th Y
+ * bin, and the last ones), anchor bins take their quadrature values, and any
+ * other bin takes the network's value only when the network's |d log L| at
+ * all four anchors around it is within max_err / 2; the rest, and every bin
+ * outside the network's domain, fall back to the quadrature.  0, 0 selects
+ * the ensemble-spread gate instead (err_scale * |r_a - r_b| <= max_err). */
+int upc_ctx_set_surrogate_anchors(upc_ctx *ctx, int every_m, int every_y);
 
 #ifdef __cplusplus
 }

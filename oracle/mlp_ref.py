@@ -7,9 +7,12 @@ L(M, Y)); this restates the library's own definition:
 Parameter vector layout (float32): W1[128][32], b1[128], W2[128][128], b2,
 W3[128][128], b3, W4[128], b4.  Loss: mean((pred - t)^2).
 
-With bf16=True the products round their operands to bf16 (round to nearest
-even) exactly where the GPU kernels do: both operands of every hidden-layer
-product (forward, dW and dH products) and W4 in the output layer.
+With bf16_ops=True the products round their operands to bf16 (round to
+nearest even) exactly where the GPU kernels do: both operands of every
+hidden-layer product (forward, dW and dH products) and W4 in the output
+layer.  With split=True (the library's default precision) every
+hidden-layer product is hi.hi + hi.lo + lo.hi, hi = bf16(x) and lo =
+bf16(x - hi) of each FP32 operand, and W4 is used unrounded.
 """
 import numpy as np
 
@@ -44,6 +47,22 @@ def _r(x, on):
     return bf16(x).astype(np.float64) if on else np.asarray(x, dtype=np.float64)
 
 
+def _hl(x):
+    x = np.asarray(x, dtype=np.float32)
+    hi = bf16(x)
+    lo = bf16(x - hi)  # exact in float32: hi is x to 8 significant bits
+    return hi.astype(np.float64), lo.astype(np.float64)
+
+
+def _mm(A, B, bf16_ops, split):
+    """A @ B with the GPU's operand rounding (A, B hold FP32 values)."""
+    if split:
+        ah, al = _hl(A)
+        bh, bl = _hl(B)
+        return ah @ bh + ah @ bl + al @ bh
+    return _r(A, bf16_ops) @ _r(B, bf16_ops)
+
+
 def _silu(z):
     return z / (1.0 + np.exp(-z))
 
@@ -53,28 +72,30 @@ def _dsilu(z):
     return s * (1.0 + z * (1.0 - s))
 
 
-def forward(p, X, bf16_ops=True):
+def forward(p, X, bf16_ops=True, split=False):
     w = unpack(p)
     X = np.asarray(X, dtype=np.float64)
-    Z1 = _r(X, bf16_ops) @ _r(w["W1"], bf16_ops).T + w["b1"]
+    m = lambda a, b: _mm(a, b, bf16_ops, split)
+    Z1 = m(X, w["W1"].T) + w["b1"]
     H1 = _silu(Z1)
-    Z2 = _r(H1, bf16_ops) @ _r(w["W2"], bf16_ops).T + w["b2"]
+    Z2 = m(H1, w["W2"].T) + w["b2"]
     H2 = _silu(Z2)
-    Z3 = _r(H2, bf16_ops) @ _r(w["W3"], bf16_ops).T + w["b3"]
+    Z3 = m(H2, w["W3"].T) + w["b3"]
     H3 = _silu(Z3)
-    pred = H3 @ _r(w["W4"], bf16_ops) + w["b4"][0]
+    pred = H3 @ _r(w["W4"], bf16_ops and not split) + w["b4"][0]
     return pred, (X, Z1, H1, Z2, H2, Z3, H3)
 
 
-def loss(p, X, T, bf16_ops=True):
-    pred, _ = forward(p, X, bf16_ops)
+def loss(p, X, T, bf16_ops=True, split=False):
+    pred, _ = forward(p, X, bf16_ops, split)
     return float(np.mean((pred - np.asarray(T, dtype=np.float64)) ** 2))
 
 
-def grad(p, X, T, bf16_ops=True):
+def grad(p, X, T, bf16_ops=True, split=False):
     """Gradient of mean((pred - T)^2) as a flat float64 vector."""
     w = unpack(p)
-    pred, (X, Z1, H1, Z2, H2, Z3, H3) = forward(p, X, bf16_ops)
+    m = lambda a, b: _mm(a, b, bf16_ops, split)
+    pred, (X, Z1, H1, Z2, H2, Z3, H3) = forward(p, X, bf16_ops, split)
     B = X.shape[0]
     gout = 2.0 * (pred - np.asarray(T, dtype=np.float64)) / B
     g = np.zeros(NPARAMS)
@@ -85,13 +106,13 @@ def grad(p, X, T, bf16_ops=True):
 
     put("W4", H3.T @ gout)
     put("b4", [gout.sum()])
-    dZ3 = gout[:, None] * _r(w["W4"], bf16_ops)[None, :] * _dsilu(Z3)
-    put("W3", _r(dZ3, bf16_ops).T @ _r(H2, bf16_ops))
+    dZ3 = gout[:, None] * _r(w["W4"], bf16_ops and not split)[None, :] * _dsilu(Z3)
+    put("W3", m(dZ3.T, H2))
     put("b3", dZ3.sum(axis=0))
-    dZ2 = (_r(dZ3, bf16_ops) @ _r(w["W3"], bf16_ops)) * _dsilu(Z2)
-    put("W2", _r(dZ2, bf16_ops).T @ _r(H1, bf16_ops))
+    dZ2 = m(dZ3, w["W3"]) * _dsilu(Z2)
+    put("W2", m(dZ2.T, H1))
     put("b2", dZ2.sum(axis=0))
-    dZ1 = (_r(dZ2, bf16_ops) @ _r(w["W2"], bf16_ops)) * _dsilu(Z1)
-    put("W1", _r(dZ1, bf16_ops).T @ _r(X, bf16_ops))
+    dZ1 = m(dZ2, w["W2"]) * _dsilu(Z1)
+    put("W1", m(dZ1.T, X))
     put("b1", dZ1.sum(axis=0))
     return g

@@ -135,12 +135,14 @@ SIGNATURES = {
     "upc_sur_set_params": (_i, [_vp, _i, _vp]),
     "upc_sur_set_norm": (_i, [_vp, _d, _d]),
     "upc_sur_get_norm": (_i, [_vp, C.POINTER(_d), C.POINTER(_d)]),
+    "upc_sur_set_precision": (_i, [_vp, _i]),
     "upc_sur_set_err_scale": (_i, [_vp, _d]),
     "upc_sur_get_err_scale": (_i, [_vp, C.POINTER(_d)]),
     "upc_sur_features": (_i, [C.POINTER(Config), _vp, _vp, _ll, _vp, _vp, _vp]),
     "upc_sur_forward": (_i, [_vp, _i, _vp, _i, _vp, _vp]),
     "upc_sur_grad": (_i, [_vp, _i, _vp, _vp, _i, _vp, _vp]),
     "upc_sur_train": (_i, [_vp, _vp, _vp, _ll, _i, _i, _d, _i, _vp, _vp]),
+    "upc_ctx_set_surrogate_anchors": (_i, [_vp, _i, _i]),
     "upc_ctx_set_surrogate": (_i, [_vp, _vp, _d]),
 }
 

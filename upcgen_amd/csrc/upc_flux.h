@@ -642,6 +642,7 @@ struct LumiView {
   // rows, and the node sum is symmetric under b1 <-> b2, so one evaluation
   // fills both; work items are the bins iy <= ny / 2.
   int mirror_out;
+  const unsigned char* need = nullptr;  // work items to run (null: all)
 };
 
 // Work items of a contraction launch.
@@ -702,7 +703,7 @@ __device__ __forceinline__ void lumi_fp64_body(const LumiView& v, int p0, LumiSm
   }
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int q = p0 + wv;
-  const bool live = q < lumi_work_items(v.npts, v.fp.mirror_ny, v.mirror_out);
+  const bool live = q < lumi_work_items(v.npts, v.fp.mirror_ny, v.mirror_out) && (!v.need || v.need[q]);
   int p = 0, pm = -1;
   if (live) lumi_work_point(v, q, &p, &pm);
   const FluxParams& fp = v.fp;

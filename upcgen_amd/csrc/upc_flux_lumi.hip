@@ -39,12 +39,16 @@ __global__ void k_grid_points(int nm, double mmin, double mmax, int ny, double y
 }
 
 // Row geometry and the number of form-factor (QAGS) nodes of each row.
+// need (may be null): rows with need[r] == 0 get no QAGS tasks (their
+// form-factor nodes are left unset).
 __global__ void k_rows(const double* __restrict__ pts, int npts, int nrows, FluxParams fp,
-                       double* __restrict__ row_k, double* __restrict__ row_logd, int* __restrict__ row_cnt)
+                       double* __restrict__ row_k, double* __restrict__ row_logd, int* __restrict__ row_cnt,
+                       const unsigned char* __restrict__ need)
 {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nrows) return;
   row_setup(pts, npts, r, fp, &row_k[r], &row_logd[r], &row_cnt[r]);
+  if (need && !need[r]) row_cnt[r] = 0;
 }
 
 // Point-charge flux for nodes with b > 2R (or FLUX_POINT 1); QAGS task ids
@@ -129,7 +133,7 @@ static long long flux_rows(int npts, int mirror_ny)
 }
 
 // Runs flux rows + QAGS for the points already in c->work.d_points.
-static int run_flux(upc_ctx* c, int npts, int mirror_ny, hipStream_t s)
+static int run_flux(upc_ctx* c, int npts, int mirror_ny, hipStream_t s, const unsigned char* row_need = nullptr)
 {
   LumiWork& w = c->work;
   const FluxParams fp = flux_params(c, mirror_ny);
@@ -138,7 +142,7 @@ static int run_flux(upc_ctx* c, int npts, int mirror_ny, hipStream_t s)
   PhaseScope ph(c, s, UPC_PHASE_FLUX);
   UPC_HIP(hipMemsetAsync(w.d_counters, 0, sizeof(unsigned long long) * 8, s));
   hipLaunchKernelGGL(k_rows, dim3(nblk(nrows, 256)), dim3(256), 0, s, w.d_points, npts, nrows, fp, w.d_row_k,
-                     w.d_row_logd, w.d_row_cnt);
+                     w.d_row_logd, w.d_row_cnt, row_need);
   const int nparts = nblk(nrows, kScanBlock);
   hipLaunchKernelGGL(k_scan_part<>, dim3(nparts), dim3(kScanBlock), 0, s, w.d_row_cnt, (long)nrows, w.d_part);
   hipLaunchKernelGGL(k_scan_top<>, dim3(1), dim3(kScanBlock), 0, s, w.d_part, nparts, w.d_counters + 4);
@@ -228,6 +232,71 @@ int lumi_points_fp64(upc_ctx* c, const double* d_pts, int npts, double* d_L, dou
     hipLaunchKernelGGL((k_lumi_fp64<kWaves, SingleLumi>), dim3(nblk(nwork, kWaves)), dim3(64 * kWaves), 0, s,
                        jr);
   }
+  UPC_HIP(hipGetLastError());
+  return read_stats(c, s);
+}
+
+// Row and work-item masks of a grid restricted to the bins with
+// need_bin != 0.  Row r < npts is bin r's k1 row and, on a symmetric grid,
+// also the k2 row of its mirror bin (im, ny - iy) (iy >= 1); rows from npts
+// on are k2 rows (of bin im * ny on a symmetric grid, of bin r - npts
+// otherwise).  A work item is needed when either of its bins is.
+__global__ void k_subset_masks(const unsigned char* __restrict__ need_bin, int npts, int nrows, int mirror_ny,
+                               int nwork, int mirror_out, unsigned char* __restrict__ row_need,
+                               unsigned char* __restrict__ item_need)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nrows) {
+    bool nd;
+    if (i < npts) {
+      nd = need_bin[i];
+      if (mirror_ny > 0) {
+        const int im = i / mirror_ny, iy = i - im * mirror_ny;
+        if (iy > 0) nd = nd || need_bin[im * mirror_ny + (mirror_ny - iy)];
+      }
+    } else {
+      nd = need_bin[mirror_ny > 0 ? (i - npts) * mirror_ny : i - npts];
+    }
+    row_need[i] = nd;
+  }
+  if (i < nwork) {
+    int p, pm;
+    work_point(i, mirror_ny, mirror_out, &p, &pm);
+    item_need[i] = need_bin[p] || (pm >= 0 && need_bin[pm]);
+  }
+}
+
+int lumi_grid_subset_fp64(upc_ctx* c, const unsigned char* d_need_bin, double* d_Lt, hipStream_t s)
+{
+  const upc_config& g = c->cfg;
+  if (g.use_polarized) return set_error("the subset grid is unpolarised"), UPC_EINVAL;
+  const int npts = g.nm * g.ny;
+  int rc = make_grid_points(c, s);
+  if (rc) return rc;
+  const int mirror_ny = grid_mirror_ny(c, true);
+  const int nrows = (int)flux_rows(npts, mirror_ny);
+  const int mirror_out = mirror_ny > 0;
+  const int nwork = lumi_work_items(npts, mirror_ny, mirror_out);
+  unsigned char *row_need = nullptr, *item_need = nullptr;
+  UPC_HIP(hipMallocAsync((void**)&row_need, (size_t)nrows, s));
+  UPC_HIP(hipMallocAsync((void**)&item_need, (size_t)nwork, s));
+  hipLaunchKernelGGL(k_subset_masks, dim3(nblk(std::max(nrows, nwork), 256)), dim3(256), 0, s, d_need_bin, npts,
+                     nrows, mirror_ny, nwork, mirror_out, row_need, item_need);
+  rc = run_flux(c, npts, mirror_ny, s, row_need);
+  if (rc) return rc;
+  const FluxParams fp = flux_params(c, mirror_ny);
+  constexpr int kWaves = 4;
+  LumiView v{c->work.d_points, c->work.d_row_logd, c->work.d_flux, fp, c->d_gaa, c->d_bk, c->d_scalars, npts,
+             g.breakup_mode, 0, d_Lt, nullptr, nullptr, mirror_out};
+  v.need = item_need;
+  SingleLumi jr{v};
+  {
+    PhaseScope ph(c, s, UPC_PHASE_LUMI);
+    hipLaunchKernelGGL((k_lumi_fp64<kWaves, SingleLumi>), dim3(nblk(nwork, kWaves)), dim3(64 * kWaves), 0, s,
+                       jr);
+  }
+  (void)hipFreeAsync(row_need, s);
+  (void)hipFreeAsync(item_need, s);
   UPC_HIP(hipGetLastError());
   return read_stats(c, s);
 }

@@ -143,6 +143,9 @@ int launch_overlap(upc_ctx* c, const double* d_b, int n, int which, double* d_ou
 int lumi_points_fp64(upc_ctx* c, const double* d_pts, int npts, double* d_L, double* d_Ls,
                      double* d_Lp, hipStream_t s);
 int make_grid_points(upc_ctx* c, hipStream_t s);
+// The FP64 grid of the bins with d_need_bin[i] != 0 only (flux rows and
+// contraction items they need; other bins of d_Lt are undefined).
+int lumi_grid_subset_fp64(upc_ctx* c, const unsigned char* d_need_bin, double* d_Lt, hipStream_t s);
 int launch_flux_point(upc_ctx* c, const double* d_b, const double* d_k, int n, double* d_out,
                       hipStream_t s);
 int launch_flux_form(upc_ctx* c, const double* d_b, const double* d_k, int n, double* d_out,

@@ -60,7 +60,8 @@ __device__ __forceinline__ unsigned short bf16_bits(float f)
   const uint32_t u = __float_as_uint(f);
   return (unsigned short)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);  // round to nearest even (finite input)
 }
-__device__ __forceinline__ float bf16_round(float f) { return __uint_as_float((uint32_t)bf16_bits(f) << 16); }
+__device__ __forceinline__ float bf16_value(unsigned short h) { return __uint_as_float((uint32_t)h << 16); }
+__device__ __forceinline__ float bf16_round(float f) { return bf16_value(bf16_bits(f)); }
 
 enum { kEpiStore = 0, kEpiBiasSilu = 1 };
 
@@ -71,7 +72,7 @@ enum { kEpiStore = 0, kEpiBiasSilu = 1 };
 // the 16x16x32 MFMA layout: lane l supplies row / column (l & 15) and k
 // values 8 (l >> 4) .. +7 of each 32-deep step, and holds C[4 (l >> 4) + i][l & 15].
 // kEpiBiasSilu: C = acc + bias[n] (pre-activation) and H = silu(C).
-template <bool TA, bool TB, int EPI>
+template <bool TA, bool TB, int EPI, bool SPLIT>
 __global__ __launch_bounds__(256) void k_sur_mm(int M, int N, int K, const float* __restrict__ A, int lda,
                                                 const float* __restrict__ B, int ldb, float* __restrict__ C,
                                                 int ldc, const float* __restrict__ bias, float* __restrict__ H)
@@ -84,7 +85,7 @@ __global__ __launch_bounds__(256) void k_sur_mm(int M, int N, int K, const float
   const int r = lane & 15, g4 = lane >> 4;
   f32x4 acc = {0.f, 0.f, 0.f, 0.f};
   for (int k0 = 0; k0 < K; k0 += 32) {
-    u16x8 a, b;
+    u16x8 a, b, al, bl;
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
       const int k = k0 + g4 * 8 + e;
@@ -92,9 +93,19 @@ __global__ __launch_bounds__(256) void k_sur_mm(int M, int N, int K, const float
       const float bv = TB ? B[(long)(n0 + r) * ldb + k] : B[(long)k * ldb + n0 + r];
       a[e] = bf16_bits(av);
       b[e] = bf16_bits(bv);
+      if (SPLIT) {  // the remainders (exact in FP32) rounded to bf16 again
+        al[e] = bf16_bits(av - bf16_value(a[e]));
+        bl[e] = bf16_bits(bv - bf16_value(b[e]));
+      }
     }
     acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b),
                                                   acc, 0, 0, 0);
+    if (SPLIT) {
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, bl),
+                                                    acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, al), __builtin_bit_cast(bf16x8, b),
+                                                    acc, 0, 0, 0);
+    }
   }
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
@@ -110,8 +121,13 @@ __global__ __launch_bounds__(256) void k_sur_mm(int M, int N, int K, const float
   }
 }
 
-// Output layer: pred[b] = sum_o bf16(W4[o]) H3[b][o] + b4; with targets, the
-// loss gradient g[b] = 2 (pred - t) / B and the summed squared error.
+// Output layer: pred[b] = sum_o w4(W4[o]) H3[b][o] + b4, w4 = bf16 rounding
+// (plain) or the FP32 weight (split); with targets, the loss gradient
+// g[b] = 2 (pred - t) / B and the summed squared error.
+template <bool SPLIT>
+__device__ __forceinline__ float out_weight(float w) { return SPLIT ? w : bf16_round(w); }
+
+template <bool SPLIT>
 __global__ __launch_bounds__(256) void k_sur_out(int B, const float* __restrict__ H3, const float* __restrict__ p,
                                                  const float* __restrict__ t, float* __restrict__ pred,
                                                  float* __restrict__ g, float* __restrict__ loss,
@@ -120,8 +136,8 @@ __global__ __launch_bounds__(256) void k_sur_out(int B, const float* __restrict_
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int b = blockIdx.x * 4 + wv;
   if (b >= B) return;
-  float s = bf16_round(p[kOffW4 + lane]) * H3[(long)b * kSurH + lane] +
-            bf16_round(p[kOffW4 + 64 + lane]) * H3[(long)b * kSurH + 64 + lane];
+  float s = out_weight<SPLIT>(p[kOffW4 + lane]) * H3[(long)b * kSurH + lane] +
+            out_weight<SPLIT>(p[kOffW4 + 64 + lane]) * H3[(long)b * kSurH + 64 + lane];
   for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
   s += p[kOffB4];
   if (lane == 0) {
@@ -134,7 +150,8 @@ __global__ __launch_bounds__(256) void k_sur_out(int B, const float* __restrict_
   }
 }
 
-// dZ3 = g[b] bf16(W4[o]) silu'(Z3)
+// dZ3 = g[b] w4(W4[o]) silu'(Z3)
+template <bool SPLIT>
 __global__ void k_sur_out_bwd(int B, const float* __restrict__ g, const float* __restrict__ p,
                               const float* __restrict__ Z, float* __restrict__ dZ)
 {
@@ -142,7 +159,7 @@ __global__ void k_sur_out_bwd(int B, const float* __restrict__ g, const float* _
   if (i >= (long)B * kSurH) return;
   const int o = (int)(i % kSurH);
   const float z = Z[i], s = 1.f / (1.f + expf(-z));
-  dZ[i] = g[i / kSurH] * bf16_round(p[kOffW4 + o]) * (s * (1.f + z * (1.f - s)));
+  dZ[i] = g[i / kSurH] * out_weight<SPLIT>(p[kOffW4 + o]) * (s * (1.f + z * (1.f - s)));
 }
 
 // dZ = dH silu'(Z), in place on dH
@@ -305,23 +322,92 @@ __global__ void k_sur_combine(long n, const float* __restrict__ pa, const float*
   }
 }
 
-__global__ void k_sur_pick(long n, const int* __restrict__ idx, const double* __restrict__ pts,
-                           double* __restrict__ MY)
+// Anchor lattice of the grid form's error check: along an axis of n bins
+// with spacing `every`, the bins 0, every, 2 every, ... and n - 1.
+__host__ __device__ __forceinline__ int anchor_count(int n, int every)
 {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int p = idx[i];
-  MY[i] = pts[4 * p];
-  MY[n + i] = pts[4 * p + 1];
+  return (n - 1) / every + 1 + ((n - 1) % every != 0 ? 1 : 0);
+}
+__host__ __device__ __forceinline__ int anchor_pos(int a, int n, int every)
+{
+  const int p = a * every;
+  return p < n ? p : n - 1;
+}
+// Anchor index of bin i on its axis, or -1.
+__device__ __forceinline__ int anchor_of(int i, int n, int every, int na)
+{
+  return i % every == 0 ? i / every : (i == n - 1 ? na - 1 : -1);
 }
 
-__global__ void k_sur_scatter(long n, const int* __restrict__ idx, const double* __restrict__ Lf,
-                              const double* __restrict__ pts, double* __restrict__ L)
+struct AnchorGrid {
+  int nm, ny, em, ey, nam, nay;
+};
+
+// (M, Y) of the anchors: M at MY[a], Y at MY[na + a].
+__global__ void k_sur_anchor_points(AnchorGrid g, const double* __restrict__ pts, double* __restrict__ MY)
+{
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  const int na = g.nam * g.nay;
+  if (a >= na) return;
+  const int am = a / g.nay, ay = a - am * g.nay;
+  const long p = (long)anchor_pos(am, g.nm, g.em) * g.ny + anchor_pos(ay, g.ny, g.ey);
+  MY[a] = pts[4 * p];
+  MY[na + a] = pts[4 * p + 1];
+}
+
+// The network's |d log L| at each anchor against the FP64 quadrature value
+// La there (infinite where the anchor lies outside the domain or either
+// value is not finite).
+__global__ void k_sur_anchor_err(AnchorGrid g, const float* __restrict__ pa, const float* __restrict__ pb,
+                                 const double* __restrict__ logL0, double mu, double sigma,
+                                 const double* __restrict__ La, double* __restrict__ aerr)
+{
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= g.nam * g.nay) return;
+  const int am = a / g.nay, ay = a - am * g.nay;
+  const long p = (long)anchor_pos(am, g.nm, g.em) * g.ny + anchor_pos(ay, g.ny, g.ey);
+  const double lg = logL0[p] + 0.5 * ((mu + sigma * pa[p]) + (mu + sigma * pb[p]));
+  const double la = La[a];
+  aerr[a] = isfinite(lg) && la > 0.0 && isfinite(la) ? fabs(lg - log(la)) : INFINITY;
+}
+
+// Grid gate with anchors: an anchor bin takes its quadrature value; any
+// other bin takes the network's value when the largest anchor error at the
+// four corners of its anchor cell is within margin * max_err, and falls
+// back to the quadrature otherwise.
+__global__ void k_sur_gate(AnchorGrid g, const float* __restrict__ pa, const float* __restrict__ pb,
+                           const double* __restrict__ logL0, double mu, double sigma,
+                           const double* __restrict__ aerr, const double* __restrict__ La, double bound,
+                           const double* __restrict__ pts, double* __restrict__ L, int* __restrict__ fallback,
+                           unsigned long long* __restrict__ nfall)
+{
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long)g.nm * g.ny) return;
+  const int im = (int)(i / g.ny), iy = (int)(i - (long)im * g.ny);
+  const double dmdy = pts[4 * i + 2] * pts[4 * i + 3];
+  const int am = anchor_of(im, g.nm, g.em, g.nam), ay = anchor_of(iy, g.ny, g.ey, g.nay);
+  if (am >= 0 && ay >= 0) {
+    L[i] = La[am * g.nay + ay] * dmdy;
+    return;
+  }
+  const int m0 = im / g.em, m1 = m0 + 1 < g.nam ? m0 + 1 : g.nam - 1;
+  const int y0 = iy / g.ey, y1 = y0 + 1 < g.nay ? y0 + 1 : g.nay - 1;
+  const double e = fmax(fmax(aerr[m0 * g.nay + y0], aerr[m0 * g.nay + y1]),
+                        fmax(aerr[m1 * g.nay + y0], aerr[m1 * g.nay + y1]));
+  const double lg = logL0[i] + 0.5 * ((mu + sigma * pa[i]) + (mu + sigma * pb[i]));
+  if (isfinite(lg) && e <= bound) {
+    L[i] = exp(lg) * dmdy;
+  } else {
+    const unsigned long long slot = atomicAdd(nfall, 1ull);
+    fallback[slot] = (int)i;
+  }
+}
+
+__global__ void k_sur_mark(long n, const int* __restrict__ idx, unsigned char* __restrict__ need)
 {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const int p = idx[i];
-  L[p] = Lf[i] * pts[4 * p + 2] * pts[4 * p + 3];
+  need[idx[i]] = 1;
 }
 
 __global__ void k_sur_take(long n, const int* __restrict__ idx, const double* __restrict__ Lg,
@@ -353,6 +439,7 @@ struct upc_sur {
   int* d_step = nullptr;
   double mu = 0, sigma = 1;
   double err_scale = 1;  // predicted error = err_scale * |r_a - r_b| (calibrated on held-out data)
+  int split = 1;         // products as split bf16 (hi hi + hi lo + lo hi), else plain bf16
   unsigned long long seed = 1;
   hipStream_t cap_stream = nullptr;
 };
@@ -361,19 +448,37 @@ namespace {
 
 float* act(upc_sur* s, int which) { return s->d_act + (long)which * kSurMaxBatch * kSurH; }
 
+// C = opA(A) opB(B) (+ bias, SiLU) in the surrogate's product precision.
+template <bool TA, bool TB, int EPI>
+void mm(const upc_sur* s, int tiles, hipStream_t st, int M, int N, int K, const float* A, int lda, const float* B,
+        int ldb, float* C, int ldc, const float* bias, float* H)
+{
+  if (s->split)
+    hipLaunchKernelGGL((k_sur_mm<TA, TB, EPI, true>), dim3(nblk(tiles, 4)), dim3(256), 0, st, M, N, K, A, lda, B,
+                       ldb, C, ldc, bias, H);
+  else
+    hipLaunchKernelGGL((k_sur_mm<TA, TB, EPI, false>), dim3(nblk(tiles, 4)), dim3(256), 0, st, M, N, K, A, lda, B,
+                       ldb, C, ldc, bias, H);
+}
+
 // forward of net `net` on B rows of s->d_xb (or X); activations in d_act.
 int forward(upc_sur* s, int net, const float* X, int B, const float* T, float* pred, float* loss, hipStream_t st)
 {
   const float* p = s->d_p + (long)net * kSurParams;
   float *Z1 = act(s, 0), *H1 = act(s, 1), *Z2 = act(s, 2), *H2 = act(s, 3), *Z3 = act(s, 4), *H3 = act(s, 5);
   const int tiles = (B / 16) * (kSurH / 16);
-  hipLaunchKernelGGL((k_sur_mm<false, true, kEpiBiasSilu>), dim3(nblk(tiles, 4)), dim3(256), 0, st, B, kSurH, kSurIn,
+  mm<false, true, kEpiBiasSilu>(s, tiles, st, B, kSurH, kSurIn,
                      X, kSurIn, p + kOffW1, kSurIn, Z1, kSurH, p + kOffB1, H1);
-  hipLaunchKernelGGL((k_sur_mm<false, true, kEpiBiasSilu>), dim3(nblk(tiles, 4)), dim3(256), 0, st, B, kSurH, kSurH,
+  mm<false, true, kEpiBiasSilu>(s, tiles, st, B, kSurH, kSurH,
                      H1, kSurH, p + kOffW2, kSurH, Z2, kSurH, p + kOffB2, H2);
-  hipLaunchKernelGGL((k_sur_mm<false, true, kEpiBiasSilu>), dim3(nblk(tiles, 4)), dim3(256), 0, st, B, kSurH, kSurH,
+  mm<false, true, kEpiBiasSilu>(s, tiles, st, B, kSurH, kSurH,
                      H2, kSurH, p + kOffW3, kSurH, Z3, kSurH, p + kOffB3, H3);
-  hipLaunchKernelGGL(k_sur_out, dim3(nblk(B, 4)), dim3(256), 0, st, B, H3, p, T, pred, s->d_gout, loss, s->d_step);
+  if (s->split)
+    hipLaunchKernelGGL(k_sur_out<true>, dim3(nblk(B, 4)), dim3(256), 0, st, B, H3, p, T, pred, s->d_gout, loss,
+                       s->d_step);
+  else
+    hipLaunchKernelGGL(k_sur_out<false>, dim3(nblk(B, 4)), dim3(256), 0, st, B, H3, p, T, pred, s->d_gout, loss,
+                       s->d_step);
   UPC_HIP(hipGetLastError());
   return 0;
 }
@@ -390,26 +495,29 @@ int backward(upc_sur* s, int net, const float* X, int B, hipStream_t st)
   const long n = (long)B * kSurH;
   hipLaunchKernelGGL(k_sur_colsum, dim3(kSurH), dim3(256), 0, st, B, kSurH, H3, s->d_gout, g + kOffW4);
   hipLaunchKernelGGL(k_sur_sum, dim3(1), dim3(256), 0, st, B, s->d_gout, g + kOffB4);
-  hipLaunchKernelGGL(k_sur_out_bwd, dim3(nblk(n, 256)), dim3(256), 0, st, B, s->d_gout, p, Z3, dA);  // dZ3
+  if (s->split)  // dZ3
+    hipLaunchKernelGGL(k_sur_out_bwd<true>, dim3(nblk(n, 256)), dim3(256), 0, st, B, s->d_gout, p, Z3, dA);
+  else
+    hipLaunchKernelGGL(k_sur_out_bwd<false>, dim3(nblk(n, 256)), dim3(256), 0, st, B, s->d_gout, p, Z3, dA);
   // layer 3: dW3 = dZ3^T H2, db3, dH2 = dZ3 W3
   const int tw = (kSurH / 16) * (kSurH / 16);
-  hipLaunchKernelGGL((k_sur_mm<true, false, kEpiStore>), dim3(nblk(tw, 4)), dim3(256), 0, st, kSurH, kSurH, B, dA,
+  mm<true, false, kEpiStore>(s, tw, st, kSurH, kSurH, B, dA,
                      kSurH, H2, kSurH, g + kOffW3, kSurH, (const float*)nullptr, (float*)nullptr);
   hipLaunchKernelGGL(k_sur_colsum, dim3(kSurH), dim3(256), 0, st, B, kSurH, dA, (const float*)nullptr, g + kOffB3);
   const int tb = (B / 16) * (kSurH / 16);
-  hipLaunchKernelGGL((k_sur_mm<false, false, kEpiStore>), dim3(nblk(tb, 4)), dim3(256), 0, st, B, kSurH, kSurH, dA,
+  mm<false, false, kEpiStore>(s, tb, st, B, kSurH, kSurH, dA,
                      kSurH, p + kOffW3, kSurH, dB, kSurH, (const float*)nullptr, (float*)nullptr);
   hipLaunchKernelGGL(k_sur_silu_bwd, dim3(nblk(n, 256)), dim3(256), 0, st, n, Z2, dB);  // dZ2
   // layer 2
-  hipLaunchKernelGGL((k_sur_mm<true, false, kEpiStore>), dim3(nblk(tw, 4)), dim3(256), 0, st, kSurH, kSurH, B, dB,
+  mm<true, false, kEpiStore>(s, tw, st, kSurH, kSurH, B, dB,
                      kSurH, H1, kSurH, g + kOffW2, kSurH, (const float*)nullptr, (float*)nullptr);
   hipLaunchKernelGGL(k_sur_colsum, dim3(kSurH), dim3(256), 0, st, B, kSurH, dB, (const float*)nullptr, g + kOffB2);
-  hipLaunchKernelGGL((k_sur_mm<false, false, kEpiStore>), dim3(nblk(tb, 4)), dim3(256), 0, st, B, kSurH, kSurH, dB,
+  mm<false, false, kEpiStore>(s, tb, st, B, kSurH, kSurH, dB,
                      kSurH, p + kOffW2, kSurH, dA, kSurH, (const float*)nullptr, (float*)nullptr);
   hipLaunchKernelGGL(k_sur_silu_bwd, dim3(nblk(n, 256)), dim3(256), 0, st, n, Z1, dA);  // dZ1
   // layer 1: dW1 = dZ1^T X [128 x 32], db1
   const int t1 = (kSurH / 16) * (kSurIn / 16);
-  hipLaunchKernelGGL((k_sur_mm<true, false, kEpiStore>), dim3(nblk(t1, 4)), dim3(256), 0, st, kSurH, kSurIn, B, dA,
+  mm<true, false, kEpiStore>(s, t1, st, kSurH, kSurIn, B, dA,
                      kSurH, X, kSurIn, g + kOffW1, kSurIn, (const float*)nullptr, (float*)nullptr);
   hipLaunchKernelGGL(k_sur_colsum, dim3(kSurH), dim3(256), 0, st, B, kSurH, dA, (const float*)nullptr, g + kOffB1);
   UPC_HIP(hipGetLastError());
@@ -461,10 +569,14 @@ void init_params(float* p, unsigned long long seed, int net)
 
 namespace upc {
 // Grid form used by upc_lumi_grid(UPC_FORM_SURROGATE).
+// Fraction of the error bound that the anchors around a bin must meet.
+constexpr double kAnchorMargin = 0.5;
+
 struct SurAttach {
   upc_sur* s;
   double max_err;
-  long long served = 0, fallback = 0;
+  int anchor_m = 8, anchor_y = 4;  // anchor spacing in M and Y bins (0: ensemble-spread gate)
+  long long served = 0, fallback = 0, anchors = 0;
 };
 
 int lumi_grid_surrogate(upc_ctx* c, double* d_L, double* d_Ls, double* d_Lp, hipStream_t st)
@@ -478,8 +590,15 @@ int lumi_grid_surrogate(upc_ctx* c, double* d_L, double* d_Ls, double* d_Lp, hip
   int rc = make_grid_points(c, st);
   if (rc) return rc;
   const long n = (long)c->cfg.nm * c->cfg.ny;
+  const bool anchored = at->anchor_m > 0 && at->anchor_y > 0;
+  AnchorGrid ag{c->cfg.nm, c->cfg.ny, at->anchor_m, at->anchor_y, 0, 0};
+  if (anchored) {
+    ag.nam = anchor_count(ag.nm, ag.em);
+    ag.nay = anchor_count(ag.ny, ag.ey);
+  }
+  const int na = ag.nam * ag.nay;
   float *X = nullptr, *pa = nullptr, *pb = nullptr;
-  double *logL0 = nullptr, *MY = nullptr, *Lf = nullptr;
+  double *logL0 = nullptr, *MY = nullptr, *Lf = nullptr, *pts = nullptr, *La = nullptr, *aerr = nullptr;
   int* idx = nullptr;
   unsigned long long* nf = nullptr;
   UPC_HIP(hipMallocAsync((void**)&X, sizeof(float) * kSurIn * n, st));
@@ -487,11 +606,14 @@ int lumi_grid_surrogate(upc_ctx* c, double* d_L, double* d_Ls, double* d_Lp, hip
   UPC_HIP(hipMallocAsync((void**)&logL0, sizeof(double) * n, st));
   UPC_HIP(hipMallocAsync((void**)&idx, sizeof(int) * n, st));
   UPC_HIP(hipMallocAsync((void**)&nf, sizeof(unsigned long long), st));
+  // the point calls below reuse the work buffer: keep the grid's (M, Y, dm, dy)
+  UPC_HIP(hipMallocAsync((void**)&pts, sizeof(double) * 4 * n, st));
+  UPC_HIP(hipMemcpyAsync(pts, c->work.d_points, sizeof(double) * 4 * n, hipMemcpyDeviceToDevice, st));
   pb = pa + n;
   {
     PhaseScope ph(c, st, UPC_PHASE_LUMI);
-    hipLaunchKernelGGL(k_sur_features, dim3(nblk(n, 256)), dim3(256), 0, st, sur_system(c->cfg),
-                       c->work.d_points, c->work.d_points + 1, n, 4, X, logL0);
+    hipLaunchKernelGGL(k_sur_features, dim3(nblk(n, 256)), dim3(256), 0, st, sur_system(c->cfg), pts, pts + 1, n,
+                       4, X, logL0);
     for (int net = 0; net < kSurNets; ++net) {
       for (long o = 0; o < n; o += kSurMaxBatch) {
         const int B = (int)std::min<long>(kSurMaxBatch, n - o);
@@ -508,44 +630,50 @@ int lumi_grid_surrogate(upc_ctx* c, double* d_L, double* d_Ls, double* d_Lp, hip
         if (rc) return rc;
       }
     }
-    UPC_HIP(hipMemsetAsync(nf, 0, sizeof(unsigned long long), st));
+  }
+  UPC_HIP(hipMemsetAsync(nf, 0, sizeof(unsigned long long), st));
+  if (anchored) {
+    // the network judged by the FP64 quadrature on the anchor lattice
+    UPC_HIP(hipMallocAsync((void**)&MY, sizeof(double) * 2 * na, st));
+    UPC_HIP(hipMallocAsync((void**)&La, sizeof(double) * na, st));
+    UPC_HIP(hipMallocAsync((void**)&aerr, sizeof(double) * na, st));
+    hipLaunchKernelGGL(k_sur_anchor_points, dim3(nblk(na, 256)), dim3(256), 0, st, ag, pts, MY);
+    rc = lumi_points_fp64(c, MY, na, La, nullptr, nullptr, st);
+    if (rc) return rc;
+    PhaseScope ph(c, st, UPC_PHASE_LUMI);
+    hipLaunchKernelGGL(k_sur_anchor_err, dim3(nblk(na, 256)), dim3(256), 0, st, ag, pa, pb, logL0, s->mu,
+                       s->sigma, La, aerr);
+    hipLaunchKernelGGL(k_sur_gate, dim3(nblk(n, 256)), dim3(256), 0, st, ag, pa, pb, logL0, s->mu, s->sigma, aerr,
+                       La, kAnchorMargin * at->max_err, pts, d_L, idx, nf);
+  } else {
+    PhaseScope ph(c, st, UPC_PHASE_LUMI);
     hipLaunchKernelGGL(k_sur_combine, dim3(nblk(n, 256)), dim3(256), 0, st, n, pa, pb, logL0, s->mu, s->sigma,
-                       s->err_scale, at->max_err, c->work.d_points, d_L, idx, nf, (double*)nullptr);
+                       s->err_scale, at->max_err, pts, d_L, idx, nf, (double*)nullptr);
   }
   unsigned long long h_nf = 0;
   UPC_HIP(hipMemcpyAsync(&h_nf, nf, sizeof(h_nf), hipMemcpyDeviceToHost, st));
   UPC_HIP(hipStreamSynchronize(st));
-  at->served = n - (long long)h_nf;
-  at->fallback = (long long)h_nf;
-  if (h_nf > (unsigned long long)(n / 2)) {
-    // most bins fall back: the grid path (mirror-shared flux rows) is cheaper
-    double* Lg = nullptr;
-    UPC_HIP(hipMallocAsync((void**)&Lg, sizeof(double) * n, st));
-    rc = lumi_points_fp64(c, nullptr, (int)n, Lg, nullptr, nullptr, st);
+  at->anchors = na;
+  at->served = n - na - (long long)h_nf;
+  at->fallback = (long long)h_nf + na;
+  if (h_nf) {
+    // the fallback bins by the FP64 grid restricted to them: their flux rows
+    // (shared between mirror bins as in the full grid) and their
+    // contraction items only
+    unsigned char* need = nullptr;
+    UPC_HIP(hipMallocAsync((void**)&need, (size_t)n, st));
+    UPC_HIP(hipMallocAsync((void**)&Lf, sizeof(double) * n, st));
+    UPC_HIP(hipMemsetAsync(need, 0, (size_t)n, st));
+    hipLaunchKernelGGL(k_sur_mark, dim3(nblk(h_nf, 256)), dim3(256), 0, st, (long)h_nf, idx, need);
+    rc = lumi_grid_subset_fp64(c, need, Lf, st);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_sur_take, dim3(nblk(h_nf, 256)), dim3(256), 0, st, (long)h_nf, idx, Lg, d_L);
-    (void)hipFreeAsync(Lg, st);
-  } else if (h_nf) {
-    // the fallback points by the FP64 quadrature (their own flux rows)
-    UPC_HIP(hipMallocAsync((void**)&MY, sizeof(double) * 2 * h_nf, st));
-    UPC_HIP(hipMallocAsync((void**)&Lf, sizeof(double) * h_nf, st));
-    hipLaunchKernelGGL(k_sur_pick, dim3(nblk(h_nf, 256)), dim3(256), 0, st, (long)h_nf, idx, c->work.d_points, MY);
-    // lumi_points_fp64 reuses the work buffer for its points: keep the grid's dm, dy first
-    double* pts_copy = nullptr;
-    UPC_HIP(hipMallocAsync((void**)&pts_copy, sizeof(double) * 4 * n, st));
-    UPC_HIP(hipMemcpyAsync(pts_copy, c->work.d_points, sizeof(double) * 4 * n, hipMemcpyDeviceToDevice, st));
-    rc = lumi_points_fp64(c, MY, (int)h_nf, Lf, nullptr, nullptr, st);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_sur_scatter, dim3(nblk(h_nf, 256)), dim3(256), 0, st, (long)h_nf, idx, Lf, pts_copy, d_L);
-    (void)hipFreeAsync(pts_copy, st);
-    (void)hipFreeAsync(MY, st);
+    hipLaunchKernelGGL(k_sur_take, dim3(nblk(h_nf, 256)), dim3(256), 0, st, (long)h_nf, idx, Lf, d_L);
+    (void)hipFreeAsync(need, st);
     (void)hipFreeAsync(Lf, st);
   }
-  (void)hipFreeAsync(X, st);
-  (void)hipFreeAsync(pa, st);
-  (void)hipFreeAsync(logL0, st);
-  (void)hipFreeAsync(idx, st);
-  (void)hipFreeAsync(nf, st);
+  void* bufs[] = {X, pa, logL0, idx, nf, pts, MY, La, aerr};
+  for (void* b : bufs)
+    if (b) (void)hipFreeAsync(b, st);
   UPC_HIP(hipGetLastError());
   return 0;
 }
@@ -638,6 +766,13 @@ int upc_sur_set_norm(upc_sur* s, double mu, double sigma)
   if (!s || !(sigma > 0)) return set_error("bad arguments"), UPC_EINVAL;
   s->mu = mu;
   s->sigma = sigma;
+  return 0;
+}
+
+int upc_sur_set_precision(upc_sur* s, int split)
+{
+  if (!s || (split != 0 && split != 1)) return set_error("bad arguments"), UPC_EINVAL;
+  s->split = split;
   return 0;
 }
 
@@ -761,6 +896,18 @@ int upc_sur_train(upc_sur* s, const float* d_X, const float* d_T, long long n, i
       std::memcpy(h_loss + (long)net * steps, tmp.data() + 1, sizeof(float) * steps);
     }
   }
+  return 0;
+}
+
+int upc_ctx_set_surrogate_anchors(upc_ctx* c, int every_m, int every_y)
+{
+  if (!c) return set_error("null context"), UPC_EINVAL;
+  auto* at = static_cast<SurAttach*>(c->surrogate);
+  if (!at) return set_error("no surrogate attached (upc_ctx_set_surrogate)"), UPC_ESTATE;
+  if (every_m < 0 || every_y < 0 || (every_m == 0) != (every_y == 0))
+    return set_error("anchor spacings must both be positive, or both 0"), UPC_EINVAL;
+  at->anchor_m = every_m;
+  at->anchor_y = every_y;
   return 0;
 }
 

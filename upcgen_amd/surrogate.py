@@ -53,6 +53,7 @@ class Surrogate:
     def __init__(self, seed: int = 1):
         _dev()
         self.seed = seed
+        self.split = True
         self._s = C.c_void_p()
         _check(load().upc_sur_create(seed, C.byref(self._s)), "upc_sur_create")
         self.nparams = param_count()
@@ -89,6 +90,11 @@ class Surrogate:
     def norm(self, v):
         _check(load().upc_sur_set_norm(self._s, float(v[0]), float(v[1])), "upc_sur_set_norm")
 
+    def set_precision(self, split: bool):
+        """Split-bf16 products (default, about FP32 products) or plain bf16."""
+        _check(load().upc_sur_set_precision(self._s, int(bool(split))), "upc_sur_set_precision")
+        self.split = bool(split)
+
     @property
     def err_scale(self):
         v = C.c_double()
@@ -99,18 +105,22 @@ class Surrogate:
     def err_scale(self, v):
         _check(load().upc_sur_set_err_scale(self._s, float(v)), "upc_sur_set_err_scale")
 
-    def calibrate(self, X, logL0, log_true, max_err=1e-3, coverage=0.99):
+    def calibrate(self, X, logL0, log_true, max_err=1e-3, coverage=0.99, min_served=0.01):
         """Error scale from held-out points: the smallest scale c such that,
         of the points the gate would serve (c * spread <= max_err), at least
-        `coverage` are within max_err.  When no scale achieves that, c is set
-        so large that nothing is served."""
+        `coverage` are within max_err, counting only scales that serve at
+        least a fraction min_served of the points (and at least 100 of them:
+        a coverage measured on a handful of points says nothing).  When no
+        scale achieves that, c is set so large that nothing is served."""
         pred, spread = self.log_lumi(X, logL0)
         err = (pred - log_true).abs().cpu().numpy()
         spr = np.maximum(spread.cpu().numpy(), 1e-300)
         order = np.argsort(spr)
         ok = np.cumsum(err[order] <= max_err) / np.arange(1, len(order) + 1)
         # serving the k smallest spreads is acceptable when the running coverage holds
+        kmin = max(100, int(np.ceil(min_served * len(order))))
         good = np.nonzero(ok >= coverage)[0]
+        good = good[good + 1 >= kmin]
         if good.size == 0:
             self.err_scale = 1e300
             return self.err_scale
@@ -170,6 +180,7 @@ class Surrogate:
     def copy(self, seed=None):
         """A new surrogate with the same weights, normalisation and error scale."""
         c = Surrogate(self.seed if seed is None else seed)
+        c.set_precision(self.split)
         for net in (0, 1):
             c.set_params(net, self.get_params(net))
         c.norm = self.norm
@@ -201,8 +212,14 @@ class Surrogate:
         s.calibrate(Xv.contiguous(), L0v, lgv, max_err=max_err, coverage=coverage)
         return s, t_label, t_train
 
-    def attach(self, ctx: Context, max_err=1e-3):
+    def attach(self, ctx: Context, max_err=1e-3, anchors=(8, 4)):
+        """Serve ctx's UPC_FORM_SURROGATE grids.  anchors: the M and Y
+        spacing of the FP64 anchor lattice that judges the network (a bin is
+        served when the network is within max_err / 2 at the four anchors
+        around it), or None for the ensemble-spread gate."""
         _check(load().upc_ctx_set_surrogate(ctx._ctx, self._s, max_err), "upc_ctx_set_surrogate")
+        em, ey = anchors if anchors else (0, 0)
+        _check(load().upc_ctx_set_surrogate_anchors(ctx._ctx, int(em), int(ey)), "upc_ctx_set_surrogate_anchors")
         ctx._surrogate = self  # keep alive while attached
 
     def save(self, path):
