@@ -45,7 +45,7 @@ FLOP_PER_QAGS_EVAL = 132
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (one
 # counter per pass) on this workload, scaled by a 1 GiB copy's readings; the
 # commands and raw counts are in profiles/pmc_traffic_1001x121_xnxn.txt.
-TRAFFIC_BYTES = {"k_flux_qags": 5.197e10, "k_lumi_fp64": 1.270e8, "k_mfma_lumi_split": 8.80e8}
+TRAFFIC_BYTES = {"k_flux_qags": 3.224e10, "k_lumi_fp64": 1.270e8, "k_mfma_lumi_split": 8.84e8}
 HBM_PEAK_GBS = 8000.0
 
 
@@ -327,8 +327,7 @@ def main():
     qags_flop = inf.qags_evals * FLOP_PER_QAGS_EVAL
     qags_tf = qags_flop / (qags_ms * 1e-3) / 1e12 if qags_ms > 0 else 0.0
     roof_qags = {"bound": "fp64 valu",
-                 "kernel": "k_flux_first + k_flux_qags (QAGS form-factor flux integrals, FP64 VALU; time and "
-                           "traffic of both launches of one grid)",
+                 "kernel": "k_flux_qags (QAGS form-factor flux integrals, one launch per grid, FP64 VALU)",
                  "achieved": round(alg_tf, 4), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                  "frac": round(alg_tf / FP64_PEAK_TFLOPS, 5), "traffic": TRAFFIC_BYTES["k_flux_qags"],
                  "traffic_unit": "bytes per launch (rocprofv3 --pmc, copy-calibrated)",

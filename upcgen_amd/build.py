@@ -1,6 +1,11 @@
 """Build libupcgen_amd.so (HIP kernels for gfx950 + C ABI + C++ facade) in-tree.
 
 Usage: python -m upcgen_amd.build   (or upcgen_amd.build.build()).
+       python -m upcgen_amd.build --resources [FILTER]
+           the compiler's resource report of every kernel (VGPRs, AGPRs,
+           SGPRs, spills, private and LDS bytes, occupancy), read from the
+           metadata of a device-only compile; FILTER keeps kernels whose
+           name contains it.
 """
 from __future__ import annotations
 
@@ -67,6 +72,48 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+def _demangled(names):
+    r = subprocess.run(["c++filt"], input="\n".join(names), capture_output=True, text=True)
+    out = r.stdout.splitlines() if r.returncode == 0 else []
+    return out if len(out) == len(names) else list(names)
+
+
+def resources(name_filter: str = "") -> list:
+    """Per-kernel resource usage from the compiler's code-object metadata."""
+    import re
+    import tempfile
+    rows = []
+    with tempfile.TemporaryDirectory() as tmp:
+        for src in HIP_SOURCES:
+            asm = os.path.join(tmp, src + ".s")
+            _run([HIPCC] + HIPFLAGS + ["--cuda-device-only", "-S", "-x", "hip", os.path.join(CSRC, src), "-o", asm])
+            text = open(asm).read()
+            for m in re.finditer(r"\n  - \.agpr_count:(.*?)(?=\n  - \.|\n\.end_amdgpu_metadata)", text, re.S):
+                block = m.group(0)
+
+                def get(key, default=0):
+                    k = re.search(r"\." + key + r":\s+(\S+)", block)
+                    return k.group(1) if k else default
+                rows.append({"source": src, "symbol": get("name", ""), "vgpr": int(get("vgpr_count")),
+                             "agpr": int(get("agpr_count")), "sgpr": int(get("sgpr_count")),
+                             "vgpr_spill": int(get("vgpr_spill_count")), "sgpr_spill": int(get("sgpr_spill_count")),
+                             "private_bytes": int(get("private_segment_fixed_size")),
+                             "lds_bytes": int(get("group_segment_fixed_size")),
+                             "waves_per_simd_by_regs": min(8, 512 // max(1, -(-(int(get("vgpr_count")) +
+                                                                            int(get("agpr_count"))) // 8) * 8))})
+    for r, d in zip(rows, _demangled([r["symbol"] for r in rows])):
+        r["kernel"] = d
+    return [r for r in rows if name_filter in r["kernel"]]
+
+
 if __name__ == "__main__":
+    if len(sys.argv) > 1 and sys.argv[1] == "--resources":
+        rows = resources(sys.argv[2] if len(sys.argv) > 2 else "")
+        print(f"{'kernel':<78} {'vgpr':>5} {'agpr':>5} {'sgpr':>5} {'v-spill':>7} {'s-spill':>7} "
+              f"{'private B':>9} {'LDS B':>7} {'waves/SIMD (regs)':>17}")
+        for r in rows:
+            print(f"{r['kernel'][:78]:<78} {r['vgpr']:>5} {r['agpr']:>5} {r['sgpr']:>5} {r['vgpr_spill']:>7} "
+                  f"{r['sgpr_spill']:>7} {r['private_bytes']:>9} {r['lds_bytes']:>7} {r['waves_per_simd_by_regs']:>17}")
+        sys.exit(0)
     build(verbose=True)
     sys.exit(0)

@@ -187,7 +187,7 @@ class Surrogate:
         c.err_scale = self.err_scale
         return c
 
-    def fit_system(self, config, n=16384, n_valid=4096, steps=4000, batch=4096, lr=3e-4, max_err=1e-3,
+    def fit_system(self, config, n=32768, n_valid=4096, steps=15000, batch=4096, lr=5e-4, max_err=1e-3,
                    coverage=0.99, seed=7):
         """A copy of this surrogate fine-tuned on one system: n FP64
         quadrature labels of `config` at uniform (M, Y) over its grid range
