@@ -158,6 +158,26 @@ def batch_bench(seed, reps):
                      "qags_ms": dict(zip(map(str, tune.qags_candidates), [round(x, 2) for x in tune.qags_ms])),
                      "lumi_ms": dict(zip(map(str, tune.lumi_candidates), [round(x, 2) for x in tune.lumi_ms]))},
            "create_s": round(t_create, 2)}
+    # the same jobs with the MFMA contraction (FP64 flux rows, split-bf16 W.D.W^T)
+    flat = lambda outs: [t.clone() for o in outs for t in (o if isinstance(o, tuple) else (o,))]
+    ref = flat(b.outputs)
+    b.set_form(U.FORM_MFMA)
+    tune = b.autotune()
+    b.run(graph=True)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(reps):
+        b.run(graph=True)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t1) * 1e3 / reps
+    got = flat(b.outputs)
+    # bins whose flux integrals stopped at a form-factor domain error are NaN in both forms
+    nan_match = all(bool((torch.isnan(g) == torch.isnan(r)).all()) for g, r in zip(got, ref))
+    err = max(float(((g - r).abs() / r.abs())[torch.isfinite(r)].max()) for g, r in zip(got, ref))
+    res["mfma"] = {"ms_per_batch": round(ms, 2), "points_per_s": round(st.points / ms * 1e3, 1),
+                   "max_rel_err_vs_fp64": err, "nan_bins": int(sum(int(torch.isnan(r).sum()) for r in ref)),
+                   "nan_bins_match": nan_match, "mfma_tile_group": tune.mfma_tile_group,
+                   "mfma_ms": dict(zip(map(str, tune.mfma_candidates), [round(x, 2) for x in tune.mfma_ms]))}
     b.close()
     return res
 

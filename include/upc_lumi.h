@@ -256,7 +256,11 @@ typedef struct upc_tune {
   int qags_candidates[3];
   double qags_ms[3];          /* QAGS kernel time of each candidate */
   int lumi_candidates[3];
-  double lumi_ms[3];          /* contraction kernel time of each candidate */
+  double lumi_ms[3];          /* FP64 contraction time of each candidate (FP64 form) */
+  int form;                   /* the batch's contraction form when tuned */
+  int mfma_tile_group;        /* chosen MFMA column tiles per A-fragment load (MFMA forms) */
+  int mfma_candidates[3];
+  double mfma_ms[3];          /* MFMA contraction time (every job) of each candidate (MFMA forms) */
 } upc_tune;
 
 typedef struct upc_batch_stats {
@@ -266,6 +270,8 @@ typedef struct upc_batch_stats {
   int qags_blocks_per_cu, lumi_waves;           /* launch shapes in use */
   long long tasks;        /* form-factor flux integrals per run (counted at creation) */
   long long work_bytes;   /* device memory of the batch's work buffers (the jobs' tables not included) */
+  int form;               /* contraction form (upc_batch_set_form) */
+  int mfma_tile_group;    /* MFMA tile shape in use */
 } upc_batch_stats;
 
 /* Builds one context (tables) per job and the batch work buffers. */
@@ -282,6 +288,15 @@ int upc_batch_run(upc_batch *batch, int use_graph, void *stream);
 int upc_batch_autotune(upc_batch *batch, void *stream, upc_tune *h_out);
 /* Sets the launch shapes directly (values from upc_tune's candidate lists). */
 int upc_batch_set_tune(upc_batch *batch, int qags_blocks_per_cu, int lumi_waves);
+/* Contraction form of every job: UPC_FORM_FP64 (default, the reference's
+ * quadrature), UPC_FORM_MFMA or UPC_FORM_MFMA_FP32 (FP64 flux rows, the
+ * shared-grid contraction of upc_lumi_grid's MFMA forms: each job's grid is
+ * the one its own context computes in that form).  The MFMA forms allocate
+ * one shared-grid state per polarisation here, not at run time. */
+int upc_batch_set_form(upc_batch *batch, int form);
+/* MFMA tile shape: column tiles that share one A-fragment load (1, 2 or 4;
+ * results are identical, only the time differs). */
+int upc_batch_set_mfma_tiles(upc_batch *batch, int tile_group);
 int upc_batch_info(upc_batch *batch, void *stream, upc_batch_stats *h_out);
 /* Per job of the last run: form-factor flux integrals that ended with a QAGS
  * error or a form-factor spline domain error (h_failed[njobs]).  The

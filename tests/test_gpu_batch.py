@@ -116,6 +116,63 @@ def test_batch_autotune(batch_and_refs):
         b.set_tune(5, 4)
 
 
+@pytest.mark.parametrize("form", [U.FORM_MFMA, U.FORM_MFMA_FP32])
+def test_batch_mfma_forms(batch_and_refs, form):
+    """The batch's MFMA forms: every job's grid is the one its own context
+    computes in that form (1e-12; point-flux jobs bit-identical), each tile
+    shape gives the same bits, graph replays equal the eager run, and the
+    grids stay within the MFMA form's accuracy of the FP64 grids (1e-5,
+    3e-5 for the 1-10 GeV Au-Au job as in the single-context MFMA tests)."""
+    import torch
+    b, cfgs, refs, _ = batch_and_refs
+    singles = []
+    for c in cfgs:
+        ctx = U.Context(c)
+        out = ctx.lumi_grid(form=form)
+        singles.append((_np(out[0]), _np(out[1])) if c.use_polarized else _np(out))
+        ctx.close()
+    try:
+        b.set_form(form)
+        first = None
+        for tg in (1, 2, 4):
+            b.set_mfma_tiles(tg)
+            eager = _grids(b.run(graph=False), cfgs)
+            torch.cuda.synchronize()
+            for o in b.outputs:
+                for t in (o if isinstance(o, tuple) else (o,)):
+                    t.fill_(-1.0)
+            replay = _grids(b.run(graph=True), cfgs)
+            torch.cuda.synchronize()
+            st = b.stats()
+            assert (st.form, st.mfma_tile_group) == (form, tg)
+            for j, (e, r, s1, ref, c) in enumerate(zip(eager, replay, singles, refs, cfgs)):
+                assert _same(e, r), (tg, j)
+                assert _close(e, s1), (tg, j)
+                if c.flux_point:
+                    assert _same(e, s1), (tg, j)
+                assert _close(e, ref, 3e-5 if j == 3 else 1e-5), (tg, j)
+            if first is None:
+                first = eager
+            else:
+                assert all(_same(g, r) for g, r in zip(eager, first)), tg
+        t = b.autotune()
+        assert t.form == form and t.mfma_tile_group in list(t.mfma_candidates)
+        assert sorted(t.mfma_candidates) == [1, 2, 4] and all(ms > 0 for ms in t.mfma_ms)
+        assert b.stats().mfma_tile_group == t.mfma_tile_group
+        got = _grids(b.run(graph=True), cfgs)
+        torch.cuda.synchronize()
+        assert all(_same(g, r) for g, r in zip(got, first))
+        with pytest.raises(U.UpcError):
+            b.set_mfma_tiles(3)
+        with pytest.raises(U.UpcError):
+            b.set_form(U.FORM_SURROGATE)
+    finally:
+        b.set_form(U.FORM_FP64)
+    got = _grids(b.run(graph=True), cfgs)
+    torch.cuda.synchronize()
+    assert all(_close(g, r) for g, r in zip(got, refs))
+
+
 def test_batch_job_against_oracle(batch_and_refs, oracle_ctx):
     b, cfgs, refs, _ = batch_and_refs
     kw = dict(JOBS[1])

@@ -76,14 +76,16 @@ PHASES = ("tables", "flux", "qags", "lumi", "xsec", "pdf", "project")
 class Tune(C.Structure):
     _fields_ = [("qags_blocks_per_cu", C.c_int), ("lumi_waves", C.c_int),
                 ("qags_candidates", C.c_int * 3), ("qags_ms", C.c_double * 3),
-                ("lumi_candidates", C.c_int * 3), ("lumi_ms", C.c_double * 3)]
+                ("lumi_candidates", C.c_int * 3), ("lumi_ms", C.c_double * 3),
+                ("form", C.c_int), ("mfma_tile_group", C.c_int),
+                ("mfma_candidates", C.c_int * 3), ("mfma_ms", C.c_double * 3)]
 
 
 class BatchStats(C.Structure):
     _fields_ = [("njobs", C.c_int), ("points", C.c_longlong), ("rows", C.c_longlong),
                 ("qags_calls", C.c_longlong), ("qags_evals", C.c_longlong), ("qags_bad", C.c_longlong),
                 ("qags_blocks_per_cu", C.c_int), ("lumi_waves", C.c_int), ("tasks", C.c_longlong),
-                ("work_bytes", C.c_longlong)]
+                ("work_bytes", C.c_longlong), ("form", C.c_int), ("mfma_tile_group", C.c_int)]
 
 
 # Every symbol declared in include/upc_lumi.h with its ctypes signature.
@@ -126,6 +128,8 @@ SIGNATURES = {
     "upc_batch_run": (_i, [_vp, _i, _vp]),
     "upc_batch_autotune": (_i, [_vp, _vp, C.POINTER(Tune)]),
     "upc_batch_set_tune": (_i, [_vp, _i, _i]),
+    "upc_batch_set_form": (_i, [_vp, _i]),
+    "upc_batch_set_mfma_tiles": (_i, [_vp, _i]),
     "upc_batch_info": (_i, [_vp, _vp, C.POINTER(BatchStats)]),
     "upc_batch_job_failures": (_i, [_vp, _vp, C.POINTER(_ll)]),
     "upc_sur_create": (_i, [_ull, C.POINTER(_vp)]),
@@ -540,6 +544,14 @@ class Batch:
 
     def set_tune(self, qags_blocks_per_cu, lumi_waves):
         _check(load().upc_batch_set_tune(self._b, qags_blocks_per_cu, lumi_waves), "upc_batch_set_tune")
+
+    def set_form(self, form):
+        """FORM_FP64 (default), FORM_MFMA or FORM_MFMA_FP32 for every job's contraction."""
+        _check(load().upc_batch_set_form(self._b, int(form)), "upc_batch_set_form")
+
+    def set_mfma_tiles(self, tile_group):
+        """MFMA column tiles per A-fragment load (1, 2 or 4; same results)."""
+        _check(load().upc_batch_set_mfma_tiles(self._b, int(tile_group)), "upc_batch_set_mfma_tiles")
 
     def job_failures(self, stream=None):
         """Per job of the last run: flux integrals that ended with a QAGS error

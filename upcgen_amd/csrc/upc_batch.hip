@@ -31,7 +31,15 @@ struct BatchJob {
   int pt0, row0;       // first point and row of the job in the batch arrays
 };
 
+// upc_mfma.hip: the MFMA contraction of one view, on a state of the batch's own
+void* mfma_state_create(double h);
+void mfma_state_free(void* st);
+int mfma_state_ready(void* st, int prec, bool pol, size_t nrows);
+int mfma_batch_contract(upc_ctx* c, void* st, const LumiView& v, int tile_group, hipStream_t s);
+void mfma_zero_ksteps(void* st, hipStream_t s);
+
 constexpr int kLumiWaveChoices[3] = {2, 4, 8};
+constexpr int kMfmaTileChoices[3] = {1, 2, 4};  // column tiles per A-fragment load (MFMA form)
 constexpr int kQagsBlockChoices[3] = {1, 2, 3};  // workgroups of 256 per CU (3 fit its LDS)
 
 __device__ __forceinline__ int find_job(const int* __restrict__ start, int njobs, long idx)
@@ -157,10 +165,13 @@ struct upc_batch {
   unsigned long long* d_counters = nullptr;  // [0] evals [1] bad [2] overflow n [4] tasks [5] cursor
   unsigned long long* d_job_failed = nullptr;  // [njobs] QAGS integrals with ier != 0 per job
   int qags_choice = 2, lumi_choice = 1;  // indices into the candidate lists
+  int form = UPC_FORM_FP64;              // contraction form of every job
+  int mfma_choice = 2;                   // index into kMfmaTileChoices
+  void* mfma[2] = {nullptr, nullptr};   // MFMA states for unpolarised / polarised jobs (largest job's size)
   hipStream_t cap_stream = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  int graph_q = -1, graph_l = -1;
+  int graph_q = -1, graph_l = -1, graph_f = -1, graph_m = -1;
   bool outputs_set = false;
 };
 
@@ -177,6 +188,8 @@ int cu_count(int dev)
 
 void free_batch(upc_batch* b)
 {
+  mfma_state_free(b->mfma[0]);
+  mfma_state_free(b->mfma[1]);
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
@@ -187,6 +200,20 @@ void free_batch(upc_batch* b)
     if (p) (void)hipFree(p);
   for (upc_ctx* c : b->ctx) upc_ctx_destroy(c);
   delete b;
+}
+
+// The MFMA contraction of every job in turn (the form's kernels per job:
+// D from the job's tables, its rows' projection, edge sums, banded GEMM),
+// on the flux rows the batch's QAGS launch filled.
+int contract_mfma(upc_batch* b, int tile_group, hipStream_t s)
+{
+  for (int k = 0; k < 2; ++k)
+    if (b->mfma[k]) mfma_zero_ksteps(b->mfma[k], s);
+  for (size_t j = 0; j < b->jobs.size(); ++j) {
+    const BatchJob& J = b->jobs[j];
+    if (int rc = mfma_batch_contract(b->ctx[j], b->mfma[J.v.pol ? 1 : 0], J.v, tile_group, s)) return rc;
+  }
+  return 0;
 }
 
 // The whole grid computation of every job (no host synchronisation).
@@ -212,6 +239,7 @@ int enqueue(upc_batch* b, hipStream_t s)
                      kQagsCap);
   hipLaunchKernelGGL((k_flux_qags_wave<BatchTask>), dim3(kQagsWaveBlocks), dim3(64), 0, s, ts, b->d_overflow,
                      b->d_counters, b->d_flux, (int*)nullptr);
+  if (b->form != UPC_FORM_FP64) return contract_mfma(b, kMfmaTileChoices[b->mfma_choice], s);
   const int li = b->lumi_choice, w = kLumiWaveChoices[li];
   BatchLumi jr{b->d_jobs, b->d_blk0[li], nj};
   if (w == 2) hipLaunchKernelGGL((k_lumi_fp64<2, BatchLumi>), dim3(b->lumi_blocks[li]), dim3(128), 0, s, jr);
@@ -387,7 +415,41 @@ int upc_batch_set_outputs(upc_batch* b, double* const* d_L, double* const* d_Ls,
   UPC_HIP(hipDeviceSynchronize());
   UPC_HIP(hipMemcpy(b->d_jobs, b->jobs.data(), sizeof(BatchJob) * b->jobs.size(), hipMemcpyHostToDevice));
   b->outputs_set = true;
+  b->graph_f = -1;  // the MFMA form's launches take the output pointers as arguments: recapture
   return 0;
+}
+
+int upc_batch_set_form(upc_batch* b, int form)
+{
+  if (!b) return set_error("null batch"), UPC_EINVAL;
+  if (form != UPC_FORM_FP64 && form != UPC_FORM_MFMA && form != UPC_FORM_MFMA_FP32)
+    return set_error("the batch runs the FP64 or an MFMA form"), UPC_EINVAL;
+  if (form != UPC_FORM_FP64) {
+    // one state per polarisation (the shared grid's extent differs), sized for its largest job
+    size_t rows[2] = {0, 0};
+    for (const BatchJob& J : b->jobs) {
+      const size_t r = J.v.fp.mirror_ny > 0 ? J.v.npts + J.v.npts / J.v.fp.mirror_ny : 2 * (size_t)J.v.npts;
+      rows[J.v.pol ? 1 : 0] = std::max(rows[J.v.pol ? 1 : 0], r);
+    }
+    for (int k = 0; k < 2; ++k) {
+      if (!rows[k]) continue;
+      if (!b->mfma[k]) b->mfma[k] = mfma_state_create(0.2);
+      if (int rc = mfma_state_ready(b->mfma[k], form, k == 1, rows[k])) return rc;
+    }
+  }
+  b->form = form;
+  return 0;
+}
+
+int upc_batch_set_mfma_tiles(upc_batch* b, int tile_group)
+{
+  if (!b) return set_error("null batch"), UPC_EINVAL;
+  for (int i = 0; i < 3; ++i)
+    if (kMfmaTileChoices[i] == tile_group) {
+      b->mfma_choice = i;
+      return 0;
+    }
+  return set_error("unsupported MFMA tile group (1, 2 or 4)"), UPC_EINVAL;
 }
 
 int upc_batch_run(upc_batch* b, int use_graph, void* stream)
@@ -396,7 +458,8 @@ int upc_batch_run(upc_batch* b, int use_graph, void* stream)
   if (!b->outputs_set) return set_error("upc_batch_set_outputs first"), UPC_ESTATE;
   hipStream_t s = (hipStream_t)stream;
   if (!use_graph) return enqueue(b, s);
-  if (!b->exec || b->graph_q != b->qags_choice || b->graph_l != b->lumi_choice) {
+  if (!b->exec || b->graph_q != b->qags_choice || b->graph_l != b->lumi_choice || b->graph_f != b->form ||
+      b->graph_m != b->mfma_choice) {
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     b->exec = nullptr;
@@ -414,6 +477,8 @@ int upc_batch_run(upc_batch* b, int use_graph, void* stream)
     UPC_HIP(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
     b->graph_q = b->qags_choice;
     b->graph_l = b->lumi_choice;
+    b->graph_f = b->form;
+    b->graph_m = b->mfma_choice;
   }
   UPC_HIP(hipGraphLaunch(b->exec, s));
   return 0;
@@ -451,7 +516,23 @@ int upc_batch_autotune(upc_batch* b, void* stream, upc_tune* h_out)
   // restore the overflow results (the timed passes re-ran every task)
   hipLaunchKernelGGL((k_flux_qags_wave<BatchTask>), dim3(kQagsWaveBlocks), dim3(64), 0, s, ts, b->d_overflow,
                      b->d_counters, b->d_flux, (int*)nullptr);
-  for (int i = 0; i < 3; ++i) {
+  for (int i = 0; i < 3 && b->form != UPC_FORM_FP64; ++i) {
+    // the MFMA form's tile shape: column tiles per A-fragment load (results identical)
+    UPC_HIP(hipEventRecord(e0, s));
+    rc = contract_mfma(b, kMfmaTileChoices[i], s);
+    if (rc) {
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      return rc;
+    }
+    UPC_HIP(hipEventRecord(e1, s));
+    UPC_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    UPC_HIP(hipEventElapsedTime(&ms, e0, e1));
+    t.mfma_candidates[i] = kMfmaTileChoices[i];
+    t.mfma_ms[i] = ms;
+  }
+  for (int i = 0; i < 3 && b->form == UPC_FORM_FP64; ++i) {
     BatchLumi jr{b->d_jobs, b->d_blk0[i], nj};
     UPC_HIP(hipEventRecord(e0, s));
     const int w = kLumiWaveChoices[i];
@@ -468,9 +549,12 @@ int upc_batch_autotune(upc_batch* b, void* stream, upc_tune* h_out)
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   b->qags_choice = (int)(std::min_element(t.qags_ms, t.qags_ms + 3) - t.qags_ms);
-  b->lumi_choice = (int)(std::min_element(t.lumi_ms, t.lumi_ms + 3) - t.lumi_ms);
+  if (b->form == UPC_FORM_FP64) b->lumi_choice = (int)(std::min_element(t.lumi_ms, t.lumi_ms + 3) - t.lumi_ms);
+  else b->mfma_choice = (int)(std::min_element(t.mfma_ms, t.mfma_ms + 3) - t.mfma_ms);
   t.qags_blocks_per_cu = kQagsBlockChoices[b->qags_choice];
   t.lumi_waves = kLumiWaveChoices[b->lumi_choice];
+  t.mfma_tile_group = kMfmaTileChoices[b->mfma_choice];
+  t.form = b->form;
   if (h_out) *h_out = t;
   return 0;
 }
@@ -515,6 +599,8 @@ int upc_batch_info(upc_batch* b, void* stream, upc_batch_stats* h_out)
   h_out->lumi_waves = kLumiWaveChoices[b->lumi_choice];
   h_out->tasks = b->ntasks;
   h_out->work_bytes = b->work_bytes;
+  h_out->form = b->form;
+  h_out->mfma_tile_group = kMfmaTileChoices[b->mfma_choice];
   return 0;
 }
 

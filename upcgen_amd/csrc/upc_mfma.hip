@@ -50,6 +50,7 @@ typedef float f32x8 __attribute__((ext_vector_type(8)));
 typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
 
 constexpr int kMfmaWaves = 4;      // waves per block in the contraction kernel
+constexpr int kMfmaTileGroup = 4;  // column tiles per A-fragment load of a single grid
 
 // e(b) of the edge split below b = 20 (x = b^2 < 400); J = edge[0],
 // -g'(20-) = edge[1], so s = -g'(20-) / 40 = edge[1] / 40.
@@ -417,7 +418,7 @@ __global__ __launch_bounds__(256) void k_mfma_edge(int npts, int mirror_out, con
 // assignment, so the sum over k is the full product); the accumulator holds
 // C[4 (l >> 4) + i][l & 15], i = 0..3.
 // ---------------------------------------------------------------------------
-template <int PREC, bool POL>
+template <int PREC, bool POL, int TG>
 __global__ __launch_bounds__(64 * kMfmaWaves) void k_mfma_lumi(
     const double* __restrict__ pts, int npts, int mirror_out, FluxParams fp, MfmaGrid g,
     const uint16_t* __restrict__ Dh,
@@ -476,9 +477,11 @@ __global__ __launch_bounds__(64 * kMfmaWaves) void k_mfma_lumi(
   // [kb0(t), kb1(t)) of the band.  kTileGroup consecutive tiles share one
   // load of each A fragment (the wave's 16 W2 rows at that k block), which
   // neighbouring tiles would otherwise each re-read; every tile still
-  // accumulates its own k blocks in ascending order.  Polarised waves carry
-  // two accumulators per tile, so they group half as many tiles.
-  constexpr int kTileGroup = POL ? 2 : 4;
+  // accumulates its own k blocks in ascending order, so the grouping (the
+  // tile shape the batch autotuner picks: 1, 2 or 4 tiles) does not change
+  // any result.  Polarised waves carry two accumulators per tile, so they
+  // group half as many tiles.
+  constexpr int kTileGroup = POL ? (TG > 1 ? TG / 2 : 1) : TG;
   auto kb0 = [&](int t) { return max(0, (t * 16 - g.band + 1) >> 5); };
   auto kb1 = [&](int t) { return min(kbmax, (t * 16 + 15 + g.band - 1) / 32 + 1); };
   for (int t0 = 0; t0 < ntiles; t0 += kTileGroup) {
@@ -583,6 +586,8 @@ __global__ __launch_bounds__(64 * kMfmaWaves) void k_mfma_lumi(
   }
 }
 
+__global__ void k_mfma_zero(unsigned long long* p) { *p = 0ull; }
+
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
@@ -670,24 +675,33 @@ int mfma_set_spacing(upc_ctx* c, double h)
 unsigned long long mfma_last_ksteps(upc_ctx* c) { return c->mfma ? state(c)->last_ksteps : 0; }
 double mfma_spacing(upc_ctx* c) { return state(c)->h; }
 
-int lumi_points_mfma(upc_ctx* c, int prec, const double* d_pts, int npts, double* d_L, double* d_Ls,
-                     double* d_Lp, hipStream_t s)
+// Buffers for views of up to nrows flux rows in precision prec; the grid
+// follows the state's spacing and polarisation.
+static int mfma_ready(MfmaState* st, int prec, bool pol, size_t nrows)
 {
-  if (npts <= 0) return 0;
-  MfmaState* st = state(c);
-  const bool pol = c->cfg.use_polarized != 0;
   const MfmaGrid g = make_grid(st->h, pol);
   if (prec != st->prec || g.nu_pad != st->g.nu_pad || g.npol != st->g.npol) free_buffers(st, false);
   st->prec = prec;
   st->g = g;
-  FluxParams fp;
-  int rc = prepare_flux(c, d_pts, npts, &fp, s);
-  if (rc) return rc;
+  return ensure_buffers(st, nrows);
+}
+
+// The MFMA contraction of one view whose flux rows are built: D from the
+// view's tables (c: the job's context), the projection of its rows, the
+// exact edge sums and the banded GEMM into the view's outputs, with
+// tile_group column tiles per A-fragment load.  The steps count into
+// st->d_ksteps (cleared by the caller).
+static int mfma_contract(upc_ctx* c, MfmaState* st, const LumiView& v, int tile_group, hipStream_t s)
+{
+  const int prec = st->prec;
+  const MfmaGrid& g = st->g;
+  const bool pol = v.pol != 0;
+  const int npts = v.npts;
+  const FluxParams& fp = v.fp;
   const int nrows = fp.mirror_ny > 0 ? npts + npts / fp.mirror_ny : 2 * npts;
-  rc = ensure_buffers(st, (size_t)nrows);
-  if (rc) return rc;
+  if ((size_t)nrows * g.nu_pad > st->rows_cap) return set_error("MFMA buffers smaller than the view"), UPC_EINVAL;
   const bool split = prec == UPC_FORM_MFMA;
-  const int mirror_out = fp.mirror_ny > 0;
+  const int mirror_out = v.mirror_out;
   const int nwork = lumi_work_items(npts, fp.mirror_ny, mirror_out);
   {
     PhaseScope ph(c, s, UPC_PHASE_PROJECT);
@@ -695,47 +709,93 @@ int lumi_points_mfma(upc_ctx* c, int prec, const double* d_pts, int npts, double
     uint16_t* Dh = split ? st->d_Dh : nullptr;
     uint16_t* Dl = split ? st->d_Dl : nullptr;
     float* D32 = split ? nullptr : st->d_D32;
-    hipLaunchKernelGGL(k_mfma_edge_params, dim3(1), dim3(1), 0, s, c->cfg.breakup_mode, c->d_gaa, c->d_bk,
-                       c->d_scalars, st->d_edge);
+    hipLaunchKernelGGL(k_mfma_edge_params, dim3(1), dim3(1), 0, s, v.mode, v.gaa, v.bk, v.sc, st->d_edge);
     if (pol)
-      hipLaunchKernelGGL(k_mfma_kernel_matrix<true>, dim3(nblk(dn, 256)), dim3(256), 0, s, g,
-                         c->cfg.breakup_mode, c->d_gaa, c->d_bk, c->d_scalars, st->d_edge, Dh, Dl, D32);
+      hipLaunchKernelGGL(k_mfma_kernel_matrix<true>, dim3(nblk(dn, 256)), dim3(256), 0, s, g, v.mode, v.gaa, v.bk,
+                         v.sc, st->d_edge, Dh, Dl, D32);
     else
-      hipLaunchKernelGGL(k_mfma_kernel_matrix<false>, dim3(nblk(dn, 256)), dim3(256), 0, s, g,
-                         c->cfg.breakup_mode, c->d_gaa, c->d_bk, c->d_scalars, st->d_edge, Dh, Dl, D32);
-    hipLaunchKernelGGL(k_mfma_project, dim3(nblk(nrows, 4)), dim3(256), 0, s, nrows, c->work.d_row_logd,
-                       c->work.d_flux, fp, g, split ? st->d_Wh : nullptr, split ? st->d_Wl : nullptr,
-                       split ? nullptr : st->d_W32, st->d_wscale, st->d_wsum, st->d_ulast);
+      hipLaunchKernelGGL(k_mfma_kernel_matrix<false>, dim3(nblk(dn, 256)), dim3(256), 0, s, g, v.mode, v.gaa,
+                         v.bk, v.sc, st->d_edge, Dh, Dl, D32);
+    hipLaunchKernelGGL(k_mfma_project, dim3(nblk(nrows, 4)), dim3(256), 0, s, nrows, v.row_logd, v.flux, fp, g,
+                       split ? st->d_Wh : nullptr, split ? st->d_Wl : nullptr, split ? nullptr : st->d_W32,
+                       st->d_wscale, st->d_wsum, st->d_ulast);
     if (pol)
-      hipLaunchKernelGGL(k_mfma_edge<true>, dim3(nblk(nwork, 4)), dim3(256), 0, s, npts, mirror_out,
-                         c->work.d_row_logd, c->work.d_flux, fp, st->d_edge, st->d_edge_sum);
+      hipLaunchKernelGGL(k_mfma_edge<true>, dim3(nblk(nwork, 4)), dim3(256), 0, s, npts, mirror_out, v.row_logd,
+                         v.flux, fp, st->d_edge, st->d_edge_sum);
     else
-      hipLaunchKernelGGL(k_mfma_edge<false>, dim3(nblk(nwork, 4)), dim3(256), 0, s, npts, mirror_out,
-                         c->work.d_row_logd, c->work.d_flux, fp, st->d_edge, st->d_edge_sum);
-    UPC_HIP(hipMemsetAsync(st->d_ksteps, 0, sizeof(unsigned long long), s));
+      hipLaunchKernelGGL(k_mfma_edge<false>, dim3(nblk(nwork, 4)), dim3(256), 0, s, npts, mirror_out, v.row_logd,
+                         v.flux, fp, st->d_edge, st->d_edge_sum);
   }
   {
     PhaseScope ph(c, s, UPC_PHASE_LUMI);
     const dim3 grid(nblk(nwork, 16 * kMfmaWaves)), block(64 * kMfmaWaves);
-#define UPC_MFMA_LAUNCH(P, POLV)                                                                   \
-  hipLaunchKernelGGL((k_mfma_lumi<P, POLV>), grid, block, 0, s, c->work.d_points, npts, mirror_out, fp, g, \
-                     st->d_Dh,                                                                     \
-                     st->d_Dl, st->d_D32, st->d_Wh, st->d_Wl, st->d_W32, st->d_wscale, st->d_wsum,      \
-                     st->d_ulast, c->cfg.breakup_mode, c->d_scalars, st->d_edge_sum, d_L, d_Ls, d_Lp,       \
-                     st->d_ksteps)
+#define UPC_MFMA_LAUNCH(P, POLV, TGV)                                                                         \
+  hipLaunchKernelGGL((k_mfma_lumi<P, POLV, TGV>), grid, block, 0, s, v.pts, npts, mirror_out, fp, g, st->d_Dh, \
+                     st->d_Dl, st->d_D32, st->d_Wh, st->d_Wl, st->d_W32, st->d_wscale, st->d_wsum,             \
+                     st->d_ulast, v.mode, v.sc, st->d_edge_sum, v.L, v.Ls, v.Lp, st->d_ksteps)
+#define UPC_MFMA_TG(P, POLV)                     \
+  do {                                           \
+    if (tile_group == 1) UPC_MFMA_LAUNCH(P, POLV, 1); \
+    else if (tile_group == 2) UPC_MFMA_LAUNCH(P, POLV, 2); \
+    else UPC_MFMA_LAUNCH(P, POLV, 4);            \
+  } while (0)
     if (split) {
-      if (pol) UPC_MFMA_LAUNCH(UPC_FORM_MFMA, true);
-      else UPC_MFMA_LAUNCH(UPC_FORM_MFMA, false);
+      if (pol) UPC_MFMA_TG(UPC_FORM_MFMA, true);
+      else UPC_MFMA_TG(UPC_FORM_MFMA, false);
     } else {
-      if (pol) UPC_MFMA_LAUNCH(UPC_FORM_MFMA_FP32, true);
-      else UPC_MFMA_LAUNCH(UPC_FORM_MFMA_FP32, false);
+      if (pol) UPC_MFMA_TG(UPC_FORM_MFMA_FP32, true);
+      else UPC_MFMA_TG(UPC_FORM_MFMA_FP32, false);
     }
+#undef UPC_MFMA_TG
 #undef UPC_MFMA_LAUNCH
   }
   UPC_HIP(hipGetLastError());
+  return 0;
+}
+
+int lumi_points_mfma(upc_ctx* c, int prec, const double* d_pts, int npts, double* d_L, double* d_Ls,
+                     double* d_Lp, hipStream_t s)
+{
+  if (npts <= 0) return 0;
+  MfmaState* st = state(c);
+  FluxParams fp;
+  int rc = prepare_flux(c, d_pts, npts, &fp, s);
+  if (rc) return rc;
+  const int nrows = fp.mirror_ny > 0 ? npts + npts / fp.mirror_ny : 2 * npts;
+  rc = mfma_ready(st, prec, c->cfg.use_polarized != 0, (size_t)nrows);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_mfma_zero, dim3(1), dim3(1), 0, s, st->d_ksteps);
+  LumiView v{c->work.d_points, c->work.d_row_logd, c->work.d_flux, fp, c->d_gaa, c->d_bk, c->d_scalars, npts,
+             c->cfg.breakup_mode, c->cfg.use_polarized, d_L, d_Ls, d_Lp, fp.mirror_ny > 0};
+  rc = mfma_contract(c, st, v, kMfmaTileGroup, s);
+  if (rc) return rc;
   UPC_HIP(hipMemcpyAsync(&st->last_ksteps, st->d_ksteps, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
   return read_flux_stats(c, s);
 }
+
+// Batch use: a state of its own (spacing h), sized for views of up to
+// nrows flux rows.
+void* mfma_state_create(double h) { auto* st = new MfmaState(); st->h = h; return st; }
+void mfma_state_free(void* p)
+{
+  if (!p) return;
+  auto* st = static_cast<MfmaState*>(p);
+  free_buffers(st, false);
+  delete st;
+}
+int mfma_state_ready(void* p, int prec, bool pol, size_t nrows)
+{
+  return mfma_ready(static_cast<MfmaState*>(p), prec, pol, nrows);
+}
+int mfma_batch_contract(upc_ctx* c, void* p, const LumiView& v, int tile_group, hipStream_t s)
+{
+  return mfma_contract(c, static_cast<MfmaState*>(p), v, tile_group, s);
+}
+void mfma_zero_ksteps(void* p, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_mfma_zero, dim3(1), dim3(1), 0, s, static_cast<MfmaState*>(p)->d_ksteps);
+}
+const unsigned long long* mfma_ksteps_ptr(void* p) { return static_cast<MfmaState*>(p)->d_ksteps; }
 
 int lumi_grid_mfma(upc_ctx* c, int prec, double* d_L, double* d_Ls, double* d_Lp, hipStream_t s)
 {
