@@ -120,6 +120,7 @@ struct BatchTask {
     ft->slot = jb;
   }
   __device__ void failed(const FluxTask& ft) const { atomicAdd(&job_failed[ft.slot], 1ull); }
+  __device__ const double4* any_ff() const { return jobs[0].ff; }
 };
 
 // Contraction job resolver: blk0[j] = first block of job j for this wave count.

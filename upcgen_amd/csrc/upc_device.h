@@ -467,13 +467,15 @@ __device__ __forceinline__ double qk21_abscissa(int e, double centr, double hlgt
 // integrand at node e, abscissa x; gx(j, side) gives back the value of
 // pair j (kXgk index), side 0 (centr - absc) or 1 (centr + absc), for the
 // mean deviation once the Kronrod mean is known.
+template <class GX>
+__device__ __forceinline__ QK21Out qk21_tail(const GX& gx, double resg, double resk, double rabs, double fc,
+                                             double hlgth);
+
 template <class FX, class GX>
 __device__ __forceinline__ QK21Out qk21_core(const FX& fx, const GX& gx, double a, double b)
 {
-  constexpr double epmach = DBL_EPSILON, uflow = DBL_MIN;
   const double centr = 0.5 * (a + b);
   const double hlgth = 0.5 * (b - a);
-  const double dhlgth = fabs(hlgth);
   double resg = 0.0, resk = 0.0, rabs = 0.0, fc = 0.0, f1 = 0.0;
 #pragma unroll 1
   for (int e = 0; e < 21; ++e) {
@@ -496,6 +498,17 @@ __device__ __forceinline__ QK21Out qk21_core(const FX& fx, const GX& gx, double 
       rabs = rabs + kWgk[xi] * (fabs(f1) + fabs(f2));
     }
   }
+  return qk21_tail(gx, resg, resk, rabs, fc, hlgth);
+}
+
+// The end of dqk21 once the node sums are known: the mean deviation (gx as
+// in qk21_core) and the error estimate.
+template <class GX>
+__device__ __forceinline__ QK21Out qk21_tail(const GX& gx, double resg, double resk, double rabs, double fc,
+                                             double hlgth)
+{
+  constexpr double epmach = DBL_EPSILON, uflow = DBL_MIN;
+  const double dhlgth = fabs(hlgth);
   const double reskh = resk * 0.5;
   double rasc = kWgk[10] * fabs(fc - reskh);
 #pragma unroll
@@ -544,11 +557,33 @@ __device__ __forceinline__ QK21Out qk21_sum(const FX& fx, double a, double b, do
       [&](int j, int side) { return fv[(2 * j + side) * stride]; }, a, b);
 }
 
-// qk21_core over values already evaluated: node e's value at v[e * stride].
+// qk21_core over values already evaluated (node e's value at v[e * stride]),
+// as two loops over pairs: the Gauss pairs (nodes 1..10), then the
+// Kronrod-only pairs (11..20), summed in qk21_core's order.  (qk21_core's
+// loop over the 21 nodes branches on each node's kind and waits for a scalar
+// load of its weights on every trip.)
 __device__ __forceinline__ QK21Out qk21_stored(const double* v, int stride, double a, double b)
 {
-  return qk21_core([&](int e, double) { return v[e * stride]; },
-                   [&](int j, int side) { return v[qk21_node(j, side) * stride]; }, a, b);
+  const double hlgth = 0.5 * (b - a);
+  const double fc = v[0];
+  double resg = 0.0, resk = kWgk[10] * fc;
+  double rabs = fabs(resk);
+#pragma unroll 1
+  for (int p = 0; p < 5; ++p) {
+    const double f1 = v[(2 * p + 1) * stride], f2 = v[(2 * p + 2) * stride];
+    const double fsum = f1 + f2;
+    resg = resg + kWg[p] * fsum;
+    resk = resk + kWgk[2 * p + 1] * fsum;
+    rabs = rabs + kWgk[2 * p + 1] * (fabs(f1) + fabs(f2));
+  }
+#pragma unroll 1
+  for (int p = 0; p < 5; ++p) {
+    const double f1 = v[(11 + 2 * p) * stride], f2 = v[(12 + 2 * p) * stride];
+    const double fsum = f1 + f2;
+    resk = resk + kWgk[2 * p] * fsum;
+    rabs = rabs + kWgk[2 * p] * (fabs(f1) + fabs(f2));
+  }
+  return qk21_tail([&](int j, int side) { return v[qk21_node(j, side) * stride]; }, resg, resk, rabs, fc, hlgth);
 }
 
 // dqk21 of f on [a, b] through qk21_sum.

@@ -173,17 +173,14 @@ struct FluxNode {
   double4 c;  // spline interval record (in flight after flux_prelude)
 };
 
-// load == false leaves the record zero and reads nothing (a caller's idle
-// lane, whose table pointer need not be valid).
-__device__ __forceinline__ void flux_prelude(const double4* __restrict__ ff, double w2g2, double q, FluxNode* n,
-                                             bool load = true)
+__device__ __forceinline__ void flux_prelude(const double4* __restrict__ ff, double w2g2, double q, FluxNode* n)
 {
   n->q = q;
   n->t = q * q + w2g2;
   n->tc = n->t < kQ2max ? n->t : kQ2max - kDQ2;
   int i = (int)((n->tc - kQ2min) * (1.0 / kDQ2));
   n->i = i < 0 ? 0 : (i > kNQ2 - 2 ? kNQ2 - 2 : i);
-  n->c = load ? ff[n->i] : make_double4(0.0, 0.0, 0.0, 0.0);
+  n->c = ff[n->i];
 }
 
 __device__ __forceinline__ double flux_finish(const FluxNode& n, double j1)
@@ -272,6 +269,7 @@ struct GridTask {  // flux-row nodes of one context
   }
   // An integral that ended with ier != 0 (counted in the launch's stats).
   __device__ void failed(const FluxTask&) const {}
+  __device__ const double4* any_ff() const { return ff; }
 };
 
 struct PointTask {  // arbitrary (b, k) points of one context
@@ -294,6 +292,7 @@ struct PointTask {  // arbitrary (b, k) points of one context
     ft->slot = 0;
   }
   __device__ void failed(const FluxTask&) const {}
+  __device__ const double4* any_ff() const { return ff; }
 };
 
 // ---------------------------------------------------------------------------
@@ -346,8 +345,12 @@ __device__ __forceinline__ int lanes_below(unsigned long long m)
 
 // Items [beg, end) of the queue with J1 branch BR, 64 per sweep.  beg, end
 // are wavefront-uniform, so every lane runs every ds_bpermute; a lane past
-// end takes a dummy item (abscissa 0) and reads no table, since its own
-// table pointer is null when it never held a task, and stores nothing.
+// end takes a dummy item (its own first node, abscissa 0) and stores
+// nothing.  The fetch has no branch: every lane reads the queue (at a
+// clamped index), its abscissa and a spline record (of a valid table: a
+// lane's table pointer is set before its first task), so the record loads
+// of a sweep's two items are issued unconditionally and each J1 waits only
+// for its own record.
 struct SweepItem {
   bool act;
   int slot;      // owner * 21 + e
@@ -359,17 +362,20 @@ template <bool SHARED_FF>
 __device__ __forceinline__ void sweep_fetch(QagsWaveLds& W, int lane, int r0, int end, const double4* ff, double b,
                                             double w2g2, SweepItem* it)
 {
+  constexpr int kItems = 21 * 64;
   const int r = r0 + lane;
   it->act = r < end;
-  const int code = it->act ? (int)W.item[r] : lane << 5;
+  const int raw = W.item[r < kItems ? r : kItems - 1];
+  const int code = it->act ? raw : lane << 5;
   const int owner = code >> 5, e = code & 31;
   const double bo = __shfl(b, owner), wo = __shfl(w2g2, owner);
   const double4* ffo = ff;
   if (!SHARED_FF) ffo = (const double4*)__shfl((long long)ff, owner);
   it->slot = owner * 21 + e;
-  const double q = it->act ? W.v[it->slot] : 0.0;
+  const double qv = W.v[it->slot];
+  const double q = it->act ? qv : 0.0;
   it->x = flux_j1_arg(bo, q);
-  flux_prelude(ffo, wo, q, &it->n, it->act);
+  flux_prelude(ffo, wo, q, &it->n);
 }
 
 template <int BR>
@@ -381,17 +387,24 @@ __device__ __forceinline__ void sweep_finish(QagsWaveLds& W, const SweepItem& it
 
 // Two items per lane in flight: both fetches (queue entry, owner's
 // parameters, abscissa, spline record load) are issued before either J1.
+// A remainder of at most 64 items takes one sweep of its own.
 template <int BR, bool SHARED_FF>
 __device__ __forceinline__ void qags_sweeps(QagsWaveLds& W, int lane, int beg, int end, const double4* ff,
                                             double b, double w2g2)
 {
+  int r0 = beg;
 #pragma unroll 1
-  for (int r0 = beg; r0 < end; r0 += 128) {
+  for (; r0 + 64 < end; r0 += 128) {
     SweepItem ia, ib;
     sweep_fetch<SHARED_FF>(W, lane, r0, end, ff, b, w2g2, &ia);
     sweep_fetch<SHARED_FF>(W, lane, r0 + 64, end, ff, b, w2g2, &ib);
     sweep_finish<BR>(W, ia);
     sweep_finish<BR>(W, ib);
+  }
+  if (r0 < end) {
+    SweepItem ia;
+    sweep_fetch<SHARED_FF>(W, lane, r0, end, ff, b, w2g2, &ia);
+    sweep_finish<BR>(W, ia);
   }
 }
 
@@ -431,12 +444,18 @@ __device__ __forceinline__ QK21Out qk21_wave(QagsWaveLds& W, int lane, bool mine
   const double centr = 0.5 * (qa + qb), hlgth = 0.5 * (qb - qa);
   double* col = W.v + lane * 21;
   wave_lds_sync();
-  int ns = 0;
+  // the abscissae of qk21_abscissa, a pair (both sides of kXgk[j]) at a time
+  col[0] = centr;
+  int ns = flux_j1_arg(b, centr) <= 8.0;
 #pragma unroll 1
-  for (int e = 0; e < 21; ++e) {
-    const double q = qk21_abscissa(e, centr, hlgth);
-    col[e] = q;
-    ns += flux_j1_arg(b, q) <= 8.0;
+  for (int j = 0; j < 10; ++j) {
+#pragma clang fp contract(off)  // absc rounded before the sums, as QUADPACK computes it
+    const double absc = hlgth * kXgk[j];
+    const double ql = centr - absc, qr = centr + absc;
+    const int e = qk21_node(j, 0);
+    col[e] = ql;
+    col[e + 1] = qr;
+    ns += (flux_j1_arg(b, ql) <= 8.0) + (flux_j1_arg(b, qr) <= 8.0);
   }
   if (!mine) ns = 0;
   int S = 0;
@@ -476,6 +495,7 @@ void k_flux_qags(TS ts, const unsigned long long* __restrict__ ntasks_p, unsigne
   const int lane = threadIdx.x & 63;
   long long task = -1;  // -1: needs a task; LLONG_MAX: exhausted
   FluxTask ft{};
+  ft.ff = ts.any_ff();  // a valid table for the sweeps' idle lanes before the first task
   double w2g2 = 0.0;
   unsigned long long my_evals = 0, my_bad = 0;
   while (true) {
