@@ -175,6 +175,7 @@ def batch_bench(seed, reps):
     nan_match = all(bool((torch.isnan(g) == torch.isnan(r)).all()) for g, r in zip(got, ref))
     err = max(float(((g - r).abs() / r.abs())[torch.isfinite(r)].max()) for g, r in zip(got, ref))
     res["mfma"] = {"ms_per_batch": round(ms, 2), "points_per_s": round(st.points / ms * 1e3, 1),
+                   "fp64_guard_items": b.stats().mfma_fp64_items,
                    "max_rel_err_vs_fp64": err, "nan_bins": int(sum(int(torch.isnan(r).sum()) for r in ref)),
                    "nan_bins_match": nan_match, "mfma_tile_group": tune.mfma_tile_group,
                    "mfma_ms": dict(zip(map(str, tune.mfma_candidates), [round(x, 2) for x in tune.mfma_ms]))}
@@ -415,6 +416,8 @@ def main():
                              "max_rel_err_vs_fp64": float(err.max()),
                              "median_rel_err_vs_fp64": float(err.median()),
                              "nan_bins": int((~ok).sum())}
+            if f in (U.FORM_MFMA, U.FORM_MFMA_FP32):
+                compare[name]["fp64_guard_items"] = ctx.info.mfma_fp64_items
         if not args.no_surrogate:
             compare["surrogate"] = surrogate_form(ctx, ref, save_path=args.surrogate_out)
         for name, v in compare.items():

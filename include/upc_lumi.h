@@ -73,6 +73,8 @@ typedef struct upc_info {
   long long surrogate_fallback;  /* bins of the last surrogate grid recomputed by quadrature */
   double bskip;          /* [fm]: below it |G_AA spline| < p20 2^-70, so the FP64 contraction
                             takes P G as 0 there (no reference counterpart) */
+  long long mfma_fp64_items; /* work items of the last MFMA grid that the cancellation guard
+                                recomputed with the FP64 contraction (upc_ctx_set_mfma_cancel_limit) */
 } upc_info;
 
 /* Phases timed with HIP events when timing is enabled (upc_ctx_set_timing). */
@@ -111,6 +113,18 @@ int upc_ctx_set_timing(upc_ctx *ctx, int enable);
  * accepted range [0.05, 2]).  No reference counterpart: the reference has
  * only the per-bin node quadrature. */
 int upc_ctx_set_mfma_spacing(upc_ctx *ctx, double h_fm);
+/* Cancellation guard of the MFMA forms (no reference counterpart).  A bin's
+ * MFMA value is F + P + E: F = s0 sum(w1) sum(w2) (FP64, the phi sum with
+ * every node beyond 20 fm), P the MFMA product of the flux rows with the
+ * kernel's difference from s0, E the exact edge part.  P's error is a small
+ * fraction (of order 1e-7) of |F|, so where P cancels most of F (flux
+ * concentrated below 2R: high M at low sqrt(s)) the bin's relative error
+ * grows with |F| / |L| (measured: about 2.5e-6 |F| / |L|).  Bins with
+ * |F| > limit |L| (default 2, which keeps every bin of the tested systems
+ * within 1e-5; for polarised grids either output) are recomputed by the
+ * FP64 contraction from the same flux rows; limit <= 0 turns the guard
+ * off.  upc_info's mfma_fp64_items counts them. */
+int upc_ctx_set_mfma_cancel_limit(upc_ctx *ctx, double limit);
 /* Intervals (1..16, default 16) a form-factor flux integral may hold in the
  * fast QAGS kernel before it is handed to the kernel with the reference's
  * 1000-interval workspace.  The result does not depend on it (both kernels
@@ -272,6 +286,8 @@ typedef struct upc_batch_stats {
   long long work_bytes;   /* device memory of the batch's work buffers (the jobs' tables not included) */
   int form;               /* contraction form (upc_batch_set_form) */
   int mfma_tile_group;    /* MFMA tile shape in use */
+  long long mfma_fp64_items; /* MFMA forms: work items of the last run recomputed in FP64 by the
+                                cancellation guard (upc_ctx_set_mfma_cancel_limit, default limit) */
 } upc_batch_stats;
 
 /* Builds one context (tables) per job and the batch work buffers. */

@@ -98,3 +98,39 @@ def test_mfma_spacing_controls_error(gpu_ctx, oracle_ctx):
     assert errs[2] < 2e-6
     with pytest.raises(U.UpcError):
         g.set_mfma_spacing(0.001)
+
+
+def test_mfma_cancellation_guard(gpu_ctx):
+    """Au-Au 200 GeV, form-factor flux, the default M and Y ranges: at high M
+    the flux product lies below 2R, where the kernel's difference from its
+    far-field value cancels most of the far-field term, and the MFMA
+    product's rounding (a fraction of that term) dominates the bin.  The
+    guard recomputes those bins with the FP64 contraction: with it every
+    bin is within MFMA_TOL of the FP64 form; with it off (limit 0) the
+    worst bins are off by more than 1e-2."""
+    kw = dict(Z=79, A=197, R=6.3691, a=0.54, sqrts=200.0, flux_point=0, breakup_mode=2, use_polarized=1,
+              nm=101, ny=41)
+    g = gpu_ctx(**kw)
+    ref = [_np(x) for x in g.lumi_grid()]
+    fin = [np.isfinite(r) for r in ref]
+
+    def errs(out):
+        for o, r, f in zip(out, ref, fin):
+            assert np.array_equal(np.isfinite(o), f)
+        return max(float(relerr(o[f], r[f]).max()) for o, r, f in zip(out, ref, fin))
+
+    for form in FORMS:
+        e = errs([_np(x) for x in g.lumi_grid(form)])
+        n = g.info.mfma_fp64_items
+        print(f"{NAMES[form]}: max rel err {e:.2e}, {n} work items recomputed in FP64")
+        assert n > 0 and e <= MFMA_TOL
+        try:
+            g.set_mfma_cancel_limit(0)
+            raw = errs([_np(x) for x in g.lumi_grid(form)])
+            assert g.info.mfma_fp64_items == 0
+            print(f"{NAMES[form]} without the guard: max rel err {raw:.2e}")
+            assert raw > 1e-2
+        finally:
+            g.set_mfma_cancel_limit(2)
+    with pytest.raises(U.UpcError):
+        g.set_mfma_cancel_limit(float("nan"))

@@ -59,7 +59,8 @@ class Info(C.Structure):
                 ("mNucl", C.c_double), ("rho0", C.c_double), ("csNN", C.c_double),
                 ("p20", C.c_double), ("qags_calls", C.c_longlong), ("qags_bad", C.c_longlong),
                 ("qags_evals", C.c_longlong), ("mfma_ksteps", C.c_longlong), ("mfma_spacing", C.c_double),
-                ("surrogate_points", C.c_longlong), ("surrogate_fallback", C.c_longlong), ("bskip", C.c_double)]
+                ("surrogate_points", C.c_longlong), ("surrogate_fallback", C.c_longlong), ("bskip", C.c_double),
+                ("mfma_fp64_items", C.c_longlong)]
 
 
 class EventOpts(C.Structure):
@@ -85,7 +86,8 @@ class BatchStats(C.Structure):
     _fields_ = [("njobs", C.c_int), ("points", C.c_longlong), ("rows", C.c_longlong),
                 ("qags_calls", C.c_longlong), ("qags_evals", C.c_longlong), ("qags_bad", C.c_longlong),
                 ("qags_blocks_per_cu", C.c_int), ("lumi_waves", C.c_int), ("tasks", C.c_longlong),
-                ("work_bytes", C.c_longlong), ("form", C.c_int), ("mfma_tile_group", C.c_int)]
+                ("work_bytes", C.c_longlong), ("form", C.c_int), ("mfma_tile_group", C.c_int),
+                ("mfma_fp64_items", C.c_longlong)]
 
 
 # Every symbol declared in include/upc_lumi.h with its ctypes signature.
@@ -129,6 +131,7 @@ SIGNATURES = {
     "upc_batch_autotune": (_i, [_vp, _vp, C.POINTER(Tune)]),
     "upc_batch_set_tune": (_i, [_vp, _i, _i]),
     "upc_batch_set_form": (_i, [_vp, _i]),
+    "upc_ctx_set_mfma_cancel_limit": (_i, [_vp, _d]),
     "upc_batch_set_mfma_tiles": (_i, [_vp, _i]),
     "upc_batch_info": (_i, [_vp, _vp, C.POINTER(BatchStats)]),
     "upc_batch_job_failures": (_i, [_vp, _vp, C.POINTER(_ll)]),
@@ -242,6 +245,11 @@ class Context:
     def set_mfma_spacing(self, h_fm):
         """Node spacing [fm] of the MFMA forms' shared b grid."""
         _check(load().upc_ctx_set_mfma_spacing(self._ctx, h_fm), "upc_ctx_set_mfma_spacing")
+
+    def set_mfma_cancel_limit(self, limit):
+        """MFMA forms: bins whose far-field term exceeds limit x |L| are
+        recomputed by the FP64 contraction (default 2; <= 0: off)."""
+        _check(load().upc_ctx_set_mfma_cancel_limit(self._ctx, float(limit)), "upc_ctx_set_mfma_cancel_limit")
 
     def set_qags_cap(self, cap):
         """Intervals the fast QAGS kernel holds before handing an integral to

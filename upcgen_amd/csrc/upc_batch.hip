@@ -37,6 +37,7 @@ void mfma_state_free(void* st);
 int mfma_state_ready(void* st, int prec, bool pol, size_t nrows);
 int mfma_batch_contract(upc_ctx* c, void* st, const LumiView& v, int tile_group, hipStream_t s);
 void mfma_zero_ksteps(void* st, hipStream_t s);
+int mfma_read_counts(void* st, unsigned long long* h_out, hipStream_t s);
 
 constexpr int kLumiWaveChoices[3] = {2, 4, 8};
 constexpr int kMfmaTileChoices[3] = {1, 2, 4};  // column tiles per A-fragment load (MFMA form)
@@ -602,6 +603,14 @@ int upc_batch_info(upc_batch* b, void* stream, upc_batch_stats* h_out)
   h_out->work_bytes = b->work_bytes;
   h_out->form = b->form;
   h_out->mfma_tile_group = kMfmaTileChoices[b->mfma_choice];
+  h_out->mfma_fp64_items = 0;
+  if (b->form != UPC_FORM_FP64)
+    for (int k = 0; k < 2; ++k) {
+      if (!b->mfma[k]) continue;
+      unsigned long long c[2];
+      if (int rc = mfma_read_counts(b->mfma[k], c, s)) return rc;
+      h_out->mfma_fp64_items += (long long)c[1];
+    }
   return 0;
 }
 

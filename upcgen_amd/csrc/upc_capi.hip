@@ -41,6 +41,8 @@ PhaseScope::~PhaseScope()
 int lumi_grid_mfma(upc_ctx* c, int prec, double* d_L, double* d_Ls, double* d_Lp, hipStream_t s);
 int mfma_set_spacing(upc_ctx* c, double h);
 unsigned long long mfma_last_ksteps(upc_ctx* c);
+unsigned long long mfma_last_fp64_items(upc_ctx* c);
+int mfma_set_cancel_limit(upc_ctx* c, double limit);
 double mfma_spacing(upc_ctx* c);
 void surrogate_counts(upc_ctx* c, long long* served, long long* fallback);
 int lumi_grid_surrogate(upc_ctx* c, double* d_L, double* d_Ls, double* d_Lp, hipStream_t s);
@@ -195,6 +197,7 @@ int upc_ctx_info(upc_ctx* c, upc_info* h)
   h->qags_bad = c->last_qags_bad;
   h->qags_evals = c->last_qags_evals;
   h->mfma_ksteps = (long long)mfma_last_ksteps(c);
+  h->mfma_fp64_items = (long long)mfma_last_fp64_items(c);
   h->mfma_spacing = mfma_spacing(c);
   surrogate_counts(c, &h->surrogate_points, &h->surrogate_fallback);
   return 0;
@@ -204,6 +207,12 @@ int upc_ctx_set_mfma_spacing(upc_ctx* c, double h_fm)
 {
   if (!c) return set_error("null context"), UPC_EINVAL;
   return mfma_set_spacing(c, h_fm);
+}
+
+int upc_ctx_set_mfma_cancel_limit(upc_ctx* c, double limit)
+{
+  if (!c) return set_error("null context"), UPC_EINVAL;
+  return mfma_set_cancel_limit(c, limit);
 }
 
 int upc_ctx_set_qags_cap(upc_ctx* c, int cap)

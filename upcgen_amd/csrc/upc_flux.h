@@ -447,7 +447,7 @@ __device__ __forceinline__ QK21Out qk21_wave(QagsWaveLds& W, int lane, bool mine
   // the abscissae of qk21_abscissa, a pair (both sides of kXgk[j]) at a time
   col[0] = centr;
   int ns = flux_j1_arg(b, centr) <= 8.0;
-#pragma unroll 1
+#pragma unroll 5
   for (int j = 0; j < 10; ++j) {
 #pragma clang fp contract(off)  // absc rounded before the sums, as QUADPACK computes it
     const double absc = hlgth * kXgk[j];
@@ -713,6 +713,11 @@ struct LumiSmem {
 template <bool POL, int WAVES>
 __device__ __forceinline__ void lumi_fp64_body(const LumiView& v, int p0, LumiSmem<WAVES>& sm)
 {
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int q = p0 + wv;
+  const bool live = q < lumi_work_items(v.npts, v.fp.mirror_ny, v.mirror_out) && (!v.need || v.need[q]);
+  // a masked launch (block-uniform v.need): blocks with no needed item end here
+  if (v.need && !__syncthreads_or(live)) return;
   for (int i = threadIdx.x; i < kNB - 1; i += blockDim.x) sm.gaa[i] = v.gaa[i];
   if (threadIdx.x < 5) {
     const double c = cos(kPi * kGLx5[threadIdx.x]);
@@ -721,9 +726,6 @@ __device__ __forceinline__ void lumi_fp64_body(const LumiView& v, int p0, LumiSm
     sm.w5[threadIdx.x] = kGLw5[threadIdx.x];
     sm.w5[5 + threadIdx.x] = sn * sn;
   }
-  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int q = p0 + wv;
-  const bool live = q < lumi_work_items(v.npts, v.fp.mirror_ny, v.mirror_out) && (!v.need || v.need[q]);
   int p = 0, pm = -1;
   if (live) lumi_work_point(v, q, &p, &pm);
   const FluxParams& fp = v.fp;
@@ -997,6 +999,9 @@ inline FluxParams flux_params(const upc_ctx* c, int mirror_ny = 0)
 // directly, form-factor nodes by the QAGS task kernels.  *fp receives the
 // row parameters (mirror_ny set when the grid's Y range is symmetric).
 int prepare_flux(upc_ctx* c, const double* d_pts, int npts, FluxParams* fp, hipStream_t s);
+// The FP64 contraction of a view (only its work items with v.need[q] != 0
+// when v.need is set) on stream s.
+int launch_lumi_fp64(const LumiView& v, hipStream_t s);
 // Copies the QAGS counters of the last prepare_flux into the context.
 int read_flux_stats(upc_ctx* c, hipStream_t s);
 

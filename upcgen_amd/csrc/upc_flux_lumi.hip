@@ -236,6 +236,17 @@ int lumi_points_fp64(upc_ctx* c, const double* d_pts, int npts, double* d_L, dou
   return read_stats(c, s);
 }
 
+int launch_lumi_fp64(const LumiView& v, hipStream_t s)
+{
+  constexpr int kWaves = 4;
+  const int nwork = lumi_work_items(v.npts, v.fp.mirror_ny, v.mirror_out);
+  if (nwork <= 0) return 0;
+  hipLaunchKernelGGL((k_lumi_fp64<kWaves, SingleLumi>), dim3(nblk(nwork, kWaves)), dim3(64 * kWaves), 0, s,
+                     SingleLumi{v});
+  UPC_HIP(hipGetLastError());
+  return 0;
+}
+
 // Row and work-item masks of a grid restricted to the bins with
 // need_bin != 0.  Row r < npts is bin r's k1 row and, on a symmetric grid,
 // also the k2 row of its mirror bin (im, ny - iy) (iy >= 1); rows from npts
