@@ -387,7 +387,10 @@ __device__ __forceinline__ void sweep_finish(QagsWaveLds& W, const SweepItem& it
 
 // Two items per lane in flight: both fetches (queue entry, owner's
 // parameters, abscissa, spline record load) are issued before either J1.
-// A remainder of at most 64 items takes one sweep of its own.
+// The J1s run at a raised wave priority, so the SIMD issues them ahead of
+// the other waves' fetches and bookkeeping, whose waits then overlap them
+// (2 % on the QAGS phase).  A remainder of at most 64 items takes one sweep
+// of its own.
 template <int BR, bool SHARED_FF>
 __device__ __forceinline__ void qags_sweeps(QagsWaveLds& W, int lane, int beg, int end, const double4* ff,
                                             double b, double w2g2)
@@ -398,8 +401,10 @@ __device__ __forceinline__ void qags_sweeps(QagsWaveLds& W, int lane, int beg, i
     SweepItem ia, ib;
     sweep_fetch<SHARED_FF>(W, lane, r0, end, ff, b, w2g2, &ia);
     sweep_fetch<SHARED_FF>(W, lane, r0 + 64, end, ff, b, w2g2, &ib);
+    __builtin_amdgcn_s_setprio(2);
     sweep_finish<BR>(W, ia);
     sweep_finish<BR>(W, ib);
+    __builtin_amdgcn_s_setprio(0);
   }
   if (r0 < end) {
     SweepItem ia;
