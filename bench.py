@@ -183,6 +183,60 @@ def batch_bench(seed, reps):
     return res
 
 
+def other_configs():
+    """The small plain-method grid (Pb-Pb 5.02 TeV, point flux, mode 1, 101x41, PROC_ID 13) and
+    end-to-end ditau events (PROC_ID 15, XNXN, form-factor flux, sigma(M,Y,z) 1001x121x100
+    from the MFMA luminosity, 10^8 Philox draws, seed 12345): GPU times beside the headline."""
+    import torch
+    import upcgen_amd as U
+    out = {}
+    c1 = U.Context(flux_point=1, breakup_mode=1, nm=101, ny=41, nz=50, proc_id=13)
+    L1 = c1.lumi_grid()
+    torch.cuda.synchronize()
+    reps = 20
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        c1.lumi_grid(out=L1)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    out["config1_101x41_point_flux"] = {"ms_per_grid": round(ms, 3), "grid_points_per_s": round(101 * 41 / ms * 1e3, 1)}
+    c1.close()
+    c5 = U.Context(flux_point=0, breakup_mode=2, nm=1001, ny=121, nz=100, proc_id=15)
+    t0 = time.perf_counter()
+    L5 = torch.nan_to_num(c5.lumi_grid(U.FORM_MFMA), nan=0.0)
+    cs, _, _ = c5.nuc_cs_ym(L5)
+    sum_ym = c5.pdf_init(cs.reshape(1, -1))[0].contiguous()
+    sum_z = c5.pdf_init(c5.cs_zm(0)).contiguous()
+    torch.cuda.synchronize()
+    t_tables = time.perf_counter() - t0
+    n_total, chunk = 100_000_000, 25_000_000
+    c5.sample_events(sum_ym, sum_z=sum_z, n=1000, seed=12345)  # warm
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for off in range(0, n_total, chunk):
+        c5.sample_events(sum_ym, sum_z=sum_z, n=chunk, seed=12345, offset=off)
+    torch.cuda.synchronize()
+    t_draw = time.perf_counter() - t0
+    c5.generate_pairs(sum_ym, sum_z=sum_z, n=1000, seed=12345)  # warm
+    torch.cuda.synchronize()
+    n_pairs = 20_000_000
+    t0 = time.perf_counter()
+    for off in range(0, n_pairs, 10_000_000):
+        c5.generate_pairs(sum_ym, sum_z=sum_z, n=10_000_000, seed=12345, offset=off)
+    torch.cuda.synchronize()
+    t_pairs = time.perf_counter() - t0
+    out["config5_ditau_events"] = {
+        "tables_s": round(t_tables, 3), "draws": n_total, "draw_s": round(t_draw, 4),
+        "draws_per_s": round(n_total / t_draw, 1),
+        "events_with_kinematics": n_pairs, "kinematics_s": round(t_pairs, 4),
+        "events_with_kinematics_per_s": round(n_pairs / t_pairs, 1),
+        "note": "(Y, M, z) draws by inverse CDF over the FP64 prefix sums; the kinematics add the "
+                "lepton 4-momenta of each pair (UpcGenerator::generateEvent with NON_ZERO_GAM_PT 0 "
+                "and no cuts)"}
+    c5.close()
+    return out
+
+
 def surrogate_form(ctx, ref, steps=20000, fit_points=32768, fit_steps=15000, save_path=None):
     """The learned surrogate as a grid form.  The general network is trained
     on 2^20 FP64 quadrature points inside its domain (256 random systems x
@@ -272,6 +326,8 @@ def main():
     ap.add_argument("--no-surrogate", action="store_true",
                     help="skip the surrogate form (training it takes about a minute)")
     ap.add_argument("--surrogate-out", default=None, help="save the system-fitted surrogate (.npz)")
+    ap.add_argument("--no-other-configs", action="store_true",
+                    help="skip the 101x41 point-flux grid and the 10^8 ditau event draws")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run three 1 GiB device-to-device copies (scale factor for the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -446,6 +502,8 @@ def main():
         "qags": {"calls": inf.qags_calls, "integrand_evals": inf.qags_evals, "bad": inf.qags_bad},
         "tot_cs_mb": tot,
     }
+    if not args.no_other_configs:
+        out["other_configs"] = other_configs()
     if args.batch:
         out["batch"] = batch_bench(args.batch_seed, max(1, min(args.steps, 3)))
     if not args.no_cpu_baseline:
