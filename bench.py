@@ -296,7 +296,7 @@ def surrogate_form(ctx, ref, steps=20000, fit_points=32768, fit_steps=15000, sav
             "served_median_rel_err": float(np.median(e[served])) if served.any() else None,
             "served_frac_within_1e-3": float(np.mean(e[served] <= 1e-3)) if served.any() else None,
             "nan_bins": int((~ok).sum()),
-            "gate": "FP64 anchors every 8 M x 4 Y bins; served where the network is within 5e-4 at the 4 "
+            "gate": "FP64 anchors every 16 M x 4 Y bins; served where the network is within 5e-4 at the 4 "
                     "anchors around the bin; outside the domain (xi > 4) and elsewhere the FP64 grid of "
                     "those bins",
             "training": {"rows": int(X.shape[0]), "steps": steps, "batch": 8192, "data_s": round(t_data, 2),

@@ -370,7 +370,7 @@ int upc_sur_train(upc_sur *sur, const float *d_X, const float *d_T, long long n,
  * scale), as are bins outside its domain.  sur = NULL detaches. */
 int upc_ctx_set_surrogate(upc_ctx *ctx, upc_sur *sur, double max_err);
 /* Error check of the surrogate grid form (no reference counterpart).  With
- * anchor spacings every_m, every_y > 0 (default 8 and 4) the FP64 quadrature
+ * anchor spacings every_m, every_y > 0 (default 16 and 4) the FP64 quadrature
  * is run at the anchor lattice (every every_m-th M bin x every every_y-th Y
  * bin, and the last ones), anchor bins take their quadrature values, and any
  * other bin takes the network's value only when the network's |d log L| at

@@ -575,7 +575,7 @@ constexpr double kAnchorMargin = 0.5;
 struct SurAttach {
   upc_sur* s;
   double max_err;
-  int anchor_m = 8, anchor_y = 4;  // anchor spacing in M and Y bins (0: ensemble-spread gate)
+  int anchor_m = 16, anchor_y = 4;  // anchor spacing in M and Y bins (0: ensemble-spread gate)
   long long served = 0, fallback = 0, anchors = 0;
 };
 

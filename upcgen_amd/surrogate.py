@@ -212,7 +212,7 @@ class Surrogate:
         s.calibrate(Xv.contiguous(), L0v, lgv, max_err=max_err, coverage=coverage)
         return s, t_label, t_train
 
-    def attach(self, ctx: Context, max_err=1e-3, anchors=(8, 4)):
+    def attach(self, ctx: Context, max_err=1e-3, anchors=(16, 4)):
         """Serve ctx's UPC_FORM_SURROGATE grids.  anchors: the M and Y
         spacing of the FP64 anchor lattice that judges the network (a bin is
         served when the network is within max_err / 2 at the four anchors
