@@ -180,3 +180,32 @@ def test_batch_job_against_oracle(batch_and_refs, oracle_ctx):
     assert nbad == 0
     got = _grids(b.run(graph=True), cfgs)[1]
     assert np.max(np.abs(got - ref) / np.abs(ref)) < 1e-9
+
+
+def test_batch_mfma_cancellation_guard(gpu):
+    """A batch with an Au-Au 200 GeV form-factor job over the full default M
+    range, where the MFMA product cancels most of the far-field term at high
+    M: the guard recomputes those work items with the FP64 contraction
+    (counted in the batch stats) and every bin of both jobs stays within
+    1e-5 of the FP64 batch."""
+    import torch
+    cfgs = [U.Config.make(Z=79, A=197, R=6.3691, a=0.54, sqrts=200.0, flux_point=0, breakup_mode=2,
+                          use_polarized=1, nm=101, ny=41),
+            U.Config.make(flux_point=0, breakup_mode=2, use_polarized=0, nm=51, ny=21)]
+    b = U.Batch(cfgs)
+    try:
+        ref = _grids(b.run(graph=False), cfgs)
+        torch.cuda.synchronize()
+        b.set_form(U.FORM_MFMA)
+        got = _grids(b.run(graph=True), cfgs)
+        torch.cuda.synchronize()
+        st = b.stats()
+        print(f"work items recomputed in FP64: {st.mfma_fp64_items}")
+        assert st.mfma_fp64_items > 0
+        for g, r in zip(got, ref):
+            for x, y in (zip(g, r) if isinstance(g, tuple) else ((g, r),)):
+                fin = np.isfinite(y)
+                assert np.array_equal(np.isfinite(x), fin)
+                assert np.max(np.abs(x[fin] - y[fin]) / np.abs(y[fin])) <= 1e-5
+    finally:
+        b.close()
