@@ -126,6 +126,7 @@ struct BatchTask {
 
 // Contraction job resolver: blk0[j] = first block of job j for this wave count.
 struct BatchLumi {
+  static constexpr int kWavesPerEu = 4;  // register budget of k_lumi_fp64 (waves per SIMD)
   const BatchJob* __restrict__ jobs;
   const int* __restrict__ blk0;
   int njobs;

@@ -697,6 +697,7 @@ __device__ __forceinline__ void lumi_work_point(const LumiView& v, int q, int* p
 
 // Job resolvers: the job and first point of a block.
 struct SingleLumi {
+  static constexpr int kWavesPerEu = 5;  // register budget of k_lumi_fp64 (waves per SIMD)
   LumiView v;
   __device__ void get(int blk, int waves, LumiView* out, int* p0) const
   {
@@ -910,12 +911,14 @@ __device__ __forceinline__ void lumi_fp64_body(const LumiView& v, int p0, LumiSm
 
 // WAVES points per block; the resolver maps the block to its job (all
 // threads of a block take the same branch, so the barriers are uniform).
-// Registers are budgeted for 5 waves per SIMD (96 VGPRs, a few spilled; the
-// LDS of 4-wave blocks allows 5 blocks per CU): the breakup-spline gathers
-// from L2 bound the kernel, and the fifth wave hides more of their latency
-// (1.58 -> 1.48 ms on the 1001 x 121 grid against 4 waves at 114 VGPRs).
+// JR::kWavesPerEu budgets the registers: the breakup-spline gathers from L2
+// bound the kernel, and for one grid a fifth wave per SIMD hides more of
+// their latency (96 VGPRs, a few spilled; the LDS of 4-wave blocks allows 5
+// blocks per CU: 1.58 -> 1.48 ms on the 1001 x 121 grid), while the batch's
+// mostly polarised jobs spill more at that budget and keep 4.
 template <int WAVES, class JR>
-__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(5))) void k_lumi_fp64(JR jr)
+__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(JR::kWavesPerEu))) void k_lumi_fp64(
+    JR jr)
 {
   __shared__ LumiSmem<WAVES> sm;
   LumiView v;
