@@ -272,12 +272,18 @@ def surrogate_form(ctx, ref, steps=20000, fit_points=32768, fit_steps=15000, sav
     L = torch.empty_like(ref)
     ctx.lumi_grid(U.FORM_SURROGATE, out=L)  # warm
     torch.cuda.synchronize()
-    reps = 3
+    reps = 5
     t2 = time.perf_counter()
     for _ in range(reps):
         ctx.lumi_grid(U.FORM_SURROGATE, out=L)
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t2) * 1e3 / reps
+    ctx.set_timing(True)
+    ctx.timings(reset=True)
+    for _ in range(reps):
+        ctx.lumi_grid(U.FORM_SURROGATE, out=L)
+    ph = ctx.timings(reset=True)
+    ctx.set_timing(False)
     inf = ctx.info
     ok = ~torch.isnan(ref)
     assert torch.equal(ok, ~torch.isnan(L)), "surrogate"
@@ -289,6 +295,8 @@ def surrogate_form(ctx, ref, steps=20000, fit_points=32768, fit_steps=15000, sav
     if save_path:
         f.save(save_path)
     return {"ms_per_grid": round(ms, 3), "grid_points_per_s": round(POINTS / ms * 1e3, 1),
+            "qags_ms": round(ph["qags"][0] / reps, 3),
+            "contraction_ms": round((ph["lumi"][0] + ph["project"][0]) / reps, 3),
             "max_rel_err_vs_fp64": float(e.max()),
             "median_rel_err_vs_fp64": float(np.median(e)),
             "served_bins": inf.surrogate_points, "fallback_bins": inf.surrogate_fallback,
@@ -459,16 +467,25 @@ def main():
                 continue
             ctx.lumi_grid(f, out=L)  # warm
             torch.cuda.synchronize()
-            reps = 3
+            reps = 5
             t2 = time.perf_counter()
             for _ in range(reps):
                 ctx.lumi_grid(f, out=L)
             torch.cuda.synchronize()
             ms = (time.perf_counter() - t2) * 1e3 / reps
+            # the same grids' phases (HIP events): the forms share the QAGS flux rows
+            ctx.set_timing(True)
+            ctx.timings(reset=True)
+            for _ in range(reps):
+                ctx.lumi_grid(f, out=L)
+            ph_f = ctx.timings(reset=True)
+            ctx.set_timing(False)
             ok = ~torch.isnan(ref)
             assert torch.equal(ok, ~torch.isnan(L)), name
             err = ((L - ref).abs() / ref.abs())[ok]
             compare[name] = {"ms_per_grid": round(ms, 3), "grid_points_per_s": round(POINTS / ms * 1e3, 1),
+                             "qags_ms": round(ph_f["qags"][0] / reps, 3),
+                             "contraction_ms": round((ph_f["lumi"][0] + ph_f["project"][0]) / reps, 3),
                              "max_rel_err_vs_fp64": float(err.max()),
                              "median_rel_err_vs_fp64": float(err.median()),
                              "nan_bins": int((~ok).sum())}
