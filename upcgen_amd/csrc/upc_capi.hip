@@ -218,7 +218,7 @@ int upc_ctx_set_mfma_cancel_limit(upc_ctx* c, double limit)
 int upc_ctx_set_qags_cap(upc_ctx* c, int cap)
 {
   if (!c) return set_error("null context"), UPC_EINVAL;
-  if (cap < 1 || cap > kQagsCap) return set_error("QAGS fast-path cap must be in [1, 16]"), UPC_EINVAL;
+  if (cap < 1 || cap > kQagsCap) return set_error("QAGS fast-path cap must be in [1, " + std::to_string(kQagsCap) + "]"), UPC_EINVAL;
   c->qags_cap = cap;
   return 0;
 }

@@ -23,7 +23,13 @@ struct DevScalars {
   int pad;
 };
 
-constexpr int kQagsCap = 16;  // fast per-lane QAGS interval list (the flux integrals stop by 15)
+// Fast per-lane QAGS interval list (the flux integrals of the 1001x121
+// Pb-Pb grid stop by 15 intervals); integrals that need more continue in the
+// 1000-interval kernel.  UPC_QAGS_FAST_CAP (2..16) sizes it at build time.
+#ifndef UPC_QAGS_FAST_CAP
+#define UPC_QAGS_FAST_CAP 16
+#endif
+constexpr int kQagsCap = UPC_QAGS_FAST_CAP;
 constexpr int kMaxEE = 1024;  // calcBreakupProb energy table capacity (reference: 10001)
 
 // Breakup knots actually needed: the luminosity reads P(min(b, 20)); the
