@@ -502,7 +502,7 @@ void k_flux_qags(TS ts, const unsigned long long* __restrict__ ntasks_p, unsigne
   FluxTask ft{};
   ft.ff = ts.any_ff();  // a valid table for the sweeps' idle lanes before the first task
   double w2g2 = 0.0;
-  unsigned long long my_evals = 0, my_bad = 0;
+  unsigned my_evals = 0, my_bad = 0;  // 32-bit per lane; evaluations flushed before they could wrap
   while (true) {
     bool need = task < 0;
     unsigned long long mask;
@@ -545,7 +545,11 @@ void k_flux_qags(TS ts, const unsigned long long* __restrict__ ntasks_p, unsigne
       flux[ft.out] = ft.factor * Q * Q / ft.k;
       const int status = q.status();
       if (ier_out) ier_out[ft.out] = status;
-      my_evals += (unsigned long long)q.neval();
+      my_evals += (unsigned)q.neval();
+      if (my_evals >= 0x80000000u) {
+        atomicAdd(&stats[0], (unsigned long long)my_evals);
+        my_evals = 0;
+      }
       my_bad += status != 0;
       if (status != 0) ts.failed(ft);
       task = -1;
@@ -555,13 +559,14 @@ void k_flux_qags(TS ts, const unsigned long long* __restrict__ ntasks_p, unsigne
       task = -1;
     }
   }
+  unsigned long long evals = my_evals, bad = my_bad;
   for (int o = 32; o > 0; o >>= 1) {
-    my_evals += __shfl_down(my_evals, o);
-    my_bad += __shfl_down(my_bad, o);
+    evals += __shfl_down(evals, o);
+    bad += __shfl_down(bad, o);
   }
   if (lane == 0) {
-    atomicAdd(&stats[0], my_evals);
-    atomicAdd(&stats[1], my_bad);
+    atomicAdd(&stats[0], evals);
+    atomicAdd(&stats[1], bad);
   }
 }
 
