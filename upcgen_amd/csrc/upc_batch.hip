@@ -120,7 +120,16 @@ struct BatchTask {
     ft->A = J.v.fp.A;
     ft->slot = jb;
   }
-  __device__ void failed(const FluxTask& ft) const { atomicAdd(&job_failed[ft.slot], 1ull); }
+  __device__ void out(long long t, int job, FluxOut* fo) const
+  {
+    const TaskRec rec = recs[t];
+    const BatchJob& J = jobs[job];
+    fo->k = rec.k;
+    fo->out = rec.out;
+    fo->factor = J.v.fp.factor;
+    fo->A = J.v.fp.A;
+  }
+  __device__ void failed(int job) const { atomicAdd(&job_failed[job], 1ull); }
   __device__ const double4* any_ff() const { return jobs[0].ff; }
 };
 

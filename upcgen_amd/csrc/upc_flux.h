@@ -249,7 +249,15 @@ struct FluxTask {
   int slot;  // job of a batched task (0 otherwise)
 };
 
-// Task sources of the QAGS kernel: get(t) describes task t.
+// What the end of an integral needs: flux[out] = factor (I / A)^2 / k.
+struct FluxOut {
+  double k, factor;
+  long out;
+  int A;
+};
+
+// Task sources of the QAGS kernel: get(t) describes task t; out(t, job)
+// gives its output fields again (job: FluxTask::slot of the same task).
 struct GridTask {  // flux-row nodes of one context
   static constexpr bool kSharedFF = true;  // one form-factor table for every task (ff)
   const TaskRec* __restrict__ recs;
@@ -267,8 +275,16 @@ struct GridTask {  // flux-row nodes of one context
     ft->A = fp.A;
     ft->slot = 0;
   }
-  // An integral that ended with ier != 0 (counted in the launch's stats).
-  __device__ void failed(const FluxTask&) const {}
+  __device__ void out(long long t, int, FluxOut* fo) const
+  {
+    const TaskRec rec = recs[t];
+    fo->k = rec.k;
+    fo->out = rec.out;
+    fo->factor = fp.factor;
+    fo->A = fp.A;
+  }
+  // An integral of job `job` that ended with ier != 0 (counted in the launch's stats).
+  __device__ void failed(int) const {}
   __device__ const double4* any_ff() const { return ff; }
 };
 
@@ -291,7 +307,15 @@ struct PointTask {  // arbitrary (b, k) points of one context
     ft->A = fp.A;
     ft->slot = 0;
   }
-  __device__ void failed(const FluxTask&) const {}
+  __device__ void out(long long t, int, FluxOut* fo) const
+  {
+    const int i = idx[t];
+    fo->k = k[i];
+    fo->out = i;
+    fo->factor = fp.factor;
+    fo->A = fp.A;
+  }
+  __device__ void failed(int) const {}
   __device__ const double4* any_ff() const { return ff; }
 };
 
@@ -499,9 +523,11 @@ void k_flux_qags(TS ts, const unsigned long long* __restrict__ ntasks_p, unsigne
   const long long ntasks = (long long)*ntasks_p;
   const int lane = threadIdx.x & 63;
   long long task = -1;  // -1: needs a task; LLONG_MAX: exhausted
-  FluxTask ft{};
-  ft.ff = ts.any_ff();  // a valid table for the sweeps' idle lanes before the first task
-  double w2g2 = 0.0;
+  // Live across the steps of a task: b, (k/g)^2, the form-factor table and the
+  // job; the output fields (k, slot, factor, A) are read again when it ends.
+  const double4* ft_ff = ts.any_ff();  // a valid table for the sweeps' idle lanes before the first task
+  double ft_b = 0.0, w2g2 = 0.0;
+  int ft_job = 0;
   unsigned my_evals = 0, my_bad = 0;  // 32-bit per lane; evaluations flushed before they could wrap
   while (true) {
     bool need = task < 0;
@@ -515,7 +541,11 @@ void k_flux_qags(TS ts, const unsigned long long* __restrict__ ntasks_p, unsigne
         const long long t = (long long)(base + (unsigned long long)lanes_below(mask));
         if (t < ntasks) {
           task = t;
+          FluxTask ft;
           ts.get(t, &ft);
+          ft_b = ft.b;
+          ft_ff = ft.ff;
+          ft_job = ft.slot;
           w2g2 = ft.k * ft.k / ft.g1 / ft.g1;
           q.start();
         } else {
@@ -529,29 +559,31 @@ void k_flux_qags(TS ts, const unsigned long long* __restrict__ ntasks_p, unsigne
     const int np = live ? q.passes() : 0;
     const double4* ff;
     if constexpr (TS::kSharedFF) ff = ts.ff;
-    else ff = ft.ff;
+    else ff = ft_ff;
     double qa = 0.0, qb = 0.0;
     if (live) q.interval(0, &qa, &qb);
-    const QK21Out o = qk21_wave<TS::kSharedFF>(W, lane, live, qa, qb, ff, ft.b, w2g2);
+    const QK21Out o = qk21_wave<TS::kSharedFF>(W, lane, live, qa, qb, ff, ft_b, w2g2);
     QK21Out o2{};
     if (__ballot(np == 2) != 0) {
       if (np == 2) q.interval(1, &qa, &qb);
-      o2 = qk21_wave<TS::kSharedFF>(W, lane, np == 2, qa, qb, ff, ft.b, w2g2);
+      o2 = qk21_wave<TS::kSharedFF>(W, lane, np == 2, qa, qb, ff, ft_b, w2g2);
     }
     if (!live) continue;
     const int st = q.consume(o, o2);
     if (st == kQagsDone) {
-      const double Q = q.result / ft.A;
-      flux[ft.out] = ft.factor * Q * Q / ft.k;
+      FluxOut fo;
+      ts.out(task, ft_job, &fo);
+      const double Q = q.result / fo.A;
+      flux[fo.out] = fo.factor * Q * Q / fo.k;
       const int status = q.status();
-      if (ier_out) ier_out[ft.out] = status;
+      if (ier_out) ier_out[fo.out] = status;
       my_evals += (unsigned)q.neval();
       if (my_evals >= 0x80000000u) {
         atomicAdd(&stats[0], (unsigned long long)my_evals);
         my_evals = 0;
       }
       my_bad += status != 0;
-      if (status != 0) ts.failed(ft);
+      if (status != 0) ts.failed(ft_job);
       task = -1;
     } else if (st == kQagsOverflow) {
       const unsigned long long slot = atomicAdd(&stats[2], 1ull);
@@ -634,7 +666,7 @@ __global__ __launch_bounds__(64) void k_flux_qags_wave(TS ts, const long long* _
       atomicAdd(&stats[0], (unsigned long long)q.neval());
       if (status != 0) {
         atomicAdd(&stats[1], 1ull);
-        ts.failed(ft);
+        ts.failed(ft.slot);
       }
     }
     __syncthreads();
