@@ -561,14 +561,16 @@ __device__ __forceinline__ QK21Out qk21_sum(const FX& fx, double a, double b, do
 // as two loops over pairs: the Gauss pairs (nodes 1..10), then the
 // Kronrod-only pairs (11..20), summed in qk21_core's order.  (qk21_core's
 // loop over the 21 nodes branches on each node's kind and waits for a scalar
-// load of its weights on every trip.)
+// load of its weights on every trip.)  The Gauss loop is unrolled (its ten
+// loads issued together); unrolling the second as well costs the QAGS kernel
+// a spilled VGPR.
 __device__ __forceinline__ QK21Out qk21_stored(const double* v, int stride, double a, double b)
 {
   const double hlgth = 0.5 * (b - a);
   const double fc = v[0];
   double resg = 0.0, resk = kWgk[10] * fc;
   double rabs = fabs(resk);
-#pragma unroll 1
+#pragma unroll
   for (int p = 0; p < 5; ++p) {
     const double f1 = v[(2 * p + 1) * stride], f2 = v[(2 * p + 2) * stride];
     const double fsum = f1 + f2;
