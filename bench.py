@@ -45,7 +45,7 @@ FLOP_PER_QAGS_EVAL = 132
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (one
 # counter per pass) on this workload, scaled by a 1 GiB copy's readings; the
 # commands and raw counts are in profiles/pmc_traffic_1001x121_xnxn.txt.
-TRAFFIC_BYTES = {"k_flux_qags": 3.233e10, "k_lumi_fp64": 1.270e8, "k_mfma_lumi_split": 8.85e8}
+TRAFFIC_BYTES = {"k_flux_qags": 3.194e10, "k_lumi_fp64": 1.270e8, "k_mfma_lumi_split": 8.85e8}
 HBM_PEAK_GBS = 8000.0
 
 
