@@ -10,7 +10,10 @@ near a bf16 rounding boundary can land one bf16 ulp (2^-8) apart; a few such
 flips among 128 products set these bounds.  Split-bf16 products (the
 default, hi.hi + hi.lo + lo.hi): such a flip moves a product by one ulp of
 its lo part, 2^-16, so forward within 1e-4 and gradients within 1e-3 in
-norm and per layer.  The baseline log L0 within 1e-10; fallback bins within
+norm and per layer.  Over 40,960 rows a few rows collect several flips:
+the forward bars then hold for 99.9 % of the rows and 10x them for all
+(the layer-by-layer kernels measured the same: max 6.6e-3 plain, 1.3e-4
+split).  The baseline log L0 within 1e-10; fallback bins within
 the FP64 form's 1e-9.
 """
 import numpy as np
@@ -52,17 +55,25 @@ def sur(gpu):
 
 
 @pytest.mark.parametrize("split", [False, True])
-def test_forward_matches_cpu_reference(sur, split):
+@pytest.mark.parametrize("rows", [256, 40960])
+def test_forward_matches_cpu_reference(sur, split, rows):
+    """Prediction is the fused inference kernel (one launch for the whole
+    network, 16 rows per wavefront); 40960 rows give the 2048 wavefronts of
+    a full launch more than one row tile each."""
     import torch
     p = _params(1)
     sur.set_precision(split)
     sur.set_params(0, p)
-    X, _ = _inputs(256, 2)
+    X, _ = _inputs(rows, 2)
     got = sur.forward(0, torch.as_tensor(X, device="cuda")).cpu().numpy()
     ref, _ = R.forward(p, X, split=split)
     err = np.abs(got - ref) / np.maximum(np.abs(ref), 1e-2)
-    print("forward max rel", err.max())
-    assert err.max() < (1e-4 if split else 1e-3)
+    print("forward max rel", err.max(), "p99.9", np.quantile(err, 0.999))
+    if rows <= 256:
+        assert err.max() < (1e-4 if split else 1e-3)
+    else:  # over many rows: the rare rows with several operand flips
+        bar = 1e-4 if split else 1e-3
+        assert np.quantile(err, 0.999) < bar and err.max() < 10 * bar
     if split:  # closer to the exact network than plain bf16 is
         exact, _ = R.forward(p, X, bf16_ops=False)
         plain, _ = R.forward(p, X)

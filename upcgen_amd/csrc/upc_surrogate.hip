@@ -150,6 +150,182 @@ __global__ __launch_bounds__(256) void k_sur_out(int B, const float* __restrict_
   }
 }
 
+// Inference of one network in one launch (no activations kept for a
+// backward pass): the three hidden layers and the output for 16 rows at a
+// time per wavefront, with the weights in LDS and the activations in
+// registers (212 VGPRs split, 169 plain; two row tiles at once would spill).
+//
+// The layers are computed transposed, C'[n][b] = sum_k W[n][k] H[b][k], so
+// the weights are the MFMA's A operand (16 output units per tile) and the
+// activations its B operand (16 rows per tile).  The 16x16x32 layout leaves
+// lane (r, g) = (l & 15, l >> 4) holding units 16 t + 4 g + i (i < 4) of row
+// r for each unit tile t; as the B operand of the next layer's k-step s the
+// lane supplies exactly those it holds, units 32 s + 4 g + j and
+// 32 s + 16 + 4 g + j (j < 4), so the weights are stored with their k index
+// in the same order and no activation crosses a lane between layers.
+// Products are split bf16 (w_hi h_hi + w_hi h_lo + w_lo h_hi) or plain bf16
+// as in k_sur_mm, with FP32 accumulation and SiLU; the output layer is FP32
+// with bf16-rounded weights (plain) or FP32 weights (split).  Only the order
+// of the FP32 sums differs from the layer-by-layer kernels.
+//
+// LDS: per layer, unit tile t and k-step s, 64 lanes x 8 bf16 (hi, then lo),
+// 147,456 B for the three layers, so one workgroup of 8 wavefronts per CU
+// (2 per SIMD); every workgroup splits and reorders the weights once.
+constexpr int kInferWaves = 8;
+constexpr int kInferTiles = 1;  // row tiles of 16 per wavefront
+
+__device__ __forceinline__ int infer_k(int s, int g, int j) { return 32 * s + 4 * g + (j < 4 ? j : 12 + j); }
+
+template <bool SPLIT>
+__global__ __launch_bounds__(64 * kInferWaves) void k_sur_infer(int B, const float* __restrict__ X,
+                                                                 const float* __restrict__ p,
+                                                                 float* __restrict__ pred)
+{
+  constexpr int kT = kSurH / 16;           // unit tiles
+  constexpr int kS = kSurH / 32;           // k-steps of layers 2 and 3
+  constexpr int kBlk = 64 * 8;             // bf16 per (tile, step) block
+  constexpr int kL1 = kT * kBlk;           // layer 1: one k-step
+  constexpr int kL23 = kT * kS * kBlk;
+  __shared__ __align__(16) unsigned short w1[2][kL1];
+  __shared__ __align__(16) unsigned short w23[2][2][kL23];  // [layer 2, 3][hi, lo]
+  __shared__ float bias[3][kSurH], w4[kSurH];
+  // weights split and reordered for the A operand
+  for (int e = threadIdx.x; e < kL1 + 2 * kL23; e += blockDim.x) {
+    const int layer = e < kL1 ? 0 : 1 + (e - kL1) / kL23;
+    const int le = layer == 0 ? e : (e - kL1) % kL23;
+    const int blk = le / kBlk, lane = (le % kBlk) >> 3, j = le & 7;
+    const int t = layer == 0 ? blk : blk / kS, s = layer == 0 ? 0 : blk % kS;
+    const int n = 16 * t + (lane & 15), g = lane >> 4;
+    const float w = layer == 0 ? p[kOffW1 + n * kSurIn + 8 * g + j]
+                               : p[(layer == 1 ? kOffW2 : kOffW3) + n * kSurH + infer_k(s, g, j)];
+    const unsigned short hi = bf16_bits(w);
+    const unsigned short lo = bf16_bits(w - bf16_value(hi));
+    if (layer == 0) {
+      w1[0][le] = hi;
+      w1[1][le] = lo;
+    } else {
+      w23[layer - 1][0][le] = hi;
+      w23[layer - 1][1][le] = lo;
+    }
+  }
+  for (int e = threadIdx.x; e < kSurH; e += blockDim.x) {
+    bias[0][e] = p[kOffB1 + e];
+    bias[1][e] = p[kOffB2 + e];
+    bias[2][e] = p[kOffB3 + e];
+    w4[e] = out_weight<SPLIT>(p[kOffW4 + e]);
+  }
+  __syncthreads();
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+  const float b4 = p[kOffB4];
+  const int ntiles = (B + 15) >> 4;
+  for (int t0 = (blockIdx.x * kInferWaves + wv) * kInferTiles; t0 < ntiles;
+       t0 += gridDim.x * kInferWaves * kInferTiles) {
+    f32x4 h[kInferTiles][kT];
+    // layer 1: B operand = the rows' 32 features, k = 8 g + j
+    {
+      bf16x8 xh[kInferTiles], xl[kInferTiles];
+#pragma unroll
+      for (int q = 0; q < kInferTiles; ++q) {
+        const int row = (t0 + q) * 16 + r;
+        u16x8 a, al;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const float x = row < B ? X[(long)row * kSurIn + 8 * g + j] : 0.f;
+          a[j] = bf16_bits(x);
+          al[j] = bf16_bits(x - bf16_value(a[j]));
+        }
+        xh[q] = __builtin_bit_cast(bf16x8, a);
+        xl[q] = __builtin_bit_cast(bf16x8, al);
+      }
+#pragma unroll
+      for (int t = 0; t < kT; ++t) {
+        const bf16x8 wa = __builtin_bit_cast(bf16x8, *(const u16x8*)&w1[0][t * kBlk + lane * 8]);
+        const bf16x8 wl = __builtin_bit_cast(bf16x8, *(const u16x8*)&w1[1][t * kBlk + lane * 8]);
+#pragma unroll
+        for (int q = 0; q < kInferTiles; ++q) {
+          f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa, xh[q], acc, 0, 0, 0);
+          if (SPLIT) {
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa, xl[q], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wl, xh[q], acc, 0, 0, 0);
+          }
+          h[q][t] = acc;
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < kInferTiles; ++q)
+#pragma unroll
+        for (int t = 0; t < kT; ++t)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const float v = h[q][t][i] + bias[0][16 * t + 4 * g + i];
+            h[q][t][i] = v / (1.f + expf(-v));
+          }
+    }
+    // layers 2 and 3
+#pragma unroll 1
+    for (int layer = 0; layer < 2; ++layer) {
+      f32x4 o[kInferTiles][kT];
+#pragma unroll
+      for (int q = 0; q < kInferTiles; ++q)
+#pragma unroll
+        for (int t = 0; t < kT; ++t) o[q][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int s = 0; s < kS; ++s) {
+        bf16x8 bh[kInferTiles], bl[kInferTiles];
+#pragma unroll
+        for (int q = 0; q < kInferTiles; ++q) {
+          u16x8 a, al;
+#pragma unroll
+          for (int j = 0; j < 8; ++j) {
+            const float x = h[q][2 * s + (j >> 2)][j & 3];
+            a[j] = bf16_bits(x);
+            al[j] = bf16_bits(x - bf16_value(a[j]));
+          }
+          bh[q] = __builtin_bit_cast(bf16x8, a);
+          bl[q] = __builtin_bit_cast(bf16x8, al);
+        }
+#pragma unroll
+        for (int t = 0; t < kT; ++t) {
+          const int off = (t * kS + s) * kBlk + lane * 8;
+          const bf16x8 wa = __builtin_bit_cast(bf16x8, *(const u16x8*)&w23[layer][0][off]);
+          const bf16x8 wl = __builtin_bit_cast(bf16x8, *(const u16x8*)&w23[layer][1][off]);
+#pragma unroll
+          for (int q = 0; q < kInferTiles; ++q) {
+            o[q][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa, bh[q], o[q][t], 0, 0, 0);
+            if (SPLIT) {
+              o[q][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa, bl[q], o[q][t], 0, 0, 0);
+              o[q][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wl, bh[q], o[q][t], 0, 0, 0);
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < kInferTiles; ++q)
+#pragma unroll
+        for (int t = 0; t < kT; ++t)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const float v = o[q][t][i] + bias[1 + layer][16 * t + 4 * g + i];
+            h[q][t][i] = v / (1.f + expf(-v));
+          }
+    }
+    // output: the lane's 32 units, then the 4 lanes of each row
+#pragma unroll
+    for (int q = 0; q < kInferTiles; ++q) {
+      float sum = 0.f;
+#pragma unroll
+      for (int t = 0; t < kT; ++t)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) sum += w4[16 * t + 4 * g + i] * h[q][t][i];
+      sum += __shfl_xor(sum, 16);
+      sum += __shfl_xor(sum, 32);
+      const int row = (t0 + q) * 16 + r;
+      if (g == 0 && row < B) pred[row] = sum + b4;
+    }
+  }
+}
+
 // dZ3 = g[b] w4(W4[o]) silu'(Z3)
 template <bool SPLIT>
 __global__ void k_sur_out_bwd(int B, const float* __restrict__ g, const float* __restrict__ p,
@@ -461,9 +637,29 @@ void mm(const upc_sur* s, int tiles, hipStream_t st, int M, int N, int K, const 
                        ldb, C, ldc, bias, H);
 }
 
+// Prediction of net `net` for B rows of X (one fused launch).
+int infer(upc_sur* s, int net, const float* X, int B, float* pred, hipStream_t st)
+{
+  if (B <= 0) return 0;
+  const float* p = s->d_p + (long)net * kSurParams;
+  int dev = 0, ncu = 0;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  const int per_blk = kInferWaves * kInferTiles;  // row tiles of a workgroup's pass
+  const int blocks = std::max(1, std::min(ncu > 0 ? ncu : 256, nblk(((long)B + 15) / 16, per_blk)));
+  if (s->split)
+    hipLaunchKernelGGL(k_sur_infer<true>, dim3(blocks), dim3(64 * kInferWaves), 0, st, B, X, p, pred);
+  else
+    hipLaunchKernelGGL(k_sur_infer<false>, dim3(blocks), dim3(64 * kInferWaves), 0, st, B, X, p, pred);
+  UPC_HIP(hipGetLastError());
+  return 0;
+}
+
 // forward of net `net` on B rows of s->d_xb (or X); activations in d_act.
+// Without targets (prediction only) it is the fused inference kernel.
 int forward(upc_sur* s, int net, const float* X, int B, const float* T, float* pred, float* loss, hipStream_t st)
 {
+  if (!T && !loss) return infer(s, net, X, B, pred, st);
   const float* p = s->d_p + (long)net * kSurParams;
   float *Z1 = act(s, 0), *H1 = act(s, 1), *Z2 = act(s, 2), *H2 = act(s, 3), *Z3 = act(s, 4), *H3 = act(s, 5);
   const int tiles = (B / 16) * (kSurH / 16);
