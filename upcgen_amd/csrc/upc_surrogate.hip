@@ -170,43 +170,64 @@ __global__ __launch_bounds__(256) void k_sur_out(int B, const float* __restrict_
 //
 // LDS: per layer, unit tile t and k-step s, 64 lanes x 8 bf16 (hi, then lo),
 // 147,456 B for the three layers, so one workgroup of 8 wavefronts per CU
-// (2 per SIMD); every workgroup splits and reorders the weights once.
+// (2 per SIMD).  k_sur_infer_prep writes that image once per call and every
+// workgroup copies it in with 16-byte loads.
 constexpr int kInferWaves = 8;
 constexpr int kInferTiles = 1;  // row tiles of 16 per wavefront
 
 __device__ __forceinline__ int infer_k(int s, int g, int j) { return 32 * s + 4 * g + (j < 4 ? j : 12 + j); }
 
+constexpr int kInferL1 = (kSurH / 16) * 64 * 8;             // layer-1 A operands (one k-step)
+constexpr int kInferL23 = (kSurH / 16) * (kSurH / 32) * 64 * 8;  // layers 2, 3
+constexpr int kInferWords = 2 * kInferL1 + 4 * kInferL23;       // bf16 hi and lo of the three layers
+
+// The A-operand image of one network's weights: per layer, unit tile t and
+// k-step s, lane (r, g) of the 16x16x32 layout holds W[16 t + r][k] for its 8
+// k (layer 1: k = 8 g + j; layers 2, 3: k = infer_k(s, g, j)), as bf16 hi
+// and lo: w1 hi, w1 lo, w2 hi, w2 lo, w3 hi, w3 lo.
+__global__ void k_sur_infer_prep(const float* __restrict__ p, unsigned short* __restrict__ img)
+{
+  constexpr int kS = kSurH / 32, kBlk = 64 * 8;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= kInferL1 + 2 * kInferL23) return;
+  const int layer = e < kInferL1 ? 0 : 1 + (e - kInferL1) / kInferL23;
+  const int le = layer == 0 ? e : (e - kInferL1) % kInferL23;
+  const int blk = le / kBlk, lane = (le % kBlk) >> 3, j = le & 7;
+  const int t = layer == 0 ? blk : blk / kS, s = layer == 0 ? 0 : blk % kS;
+  const int n = 16 * t + (lane & 15), g = lane >> 4;
+  const float w = layer == 0 ? p[kOffW1 + n * kSurIn + 8 * g + j]
+                             : p[(layer == 1 ? kOffW2 : kOffW3) + n * kSurH + infer_k(s, g, j)];
+  const unsigned short hi = bf16_bits(w);
+  const unsigned short lo = bf16_bits(w - bf16_value(hi));
+  unsigned short* base = layer == 0 ? img : img + 2 * kInferL1 + 2 * (layer - 1) * kInferL23;
+  const int half = layer == 0 ? kInferL1 : kInferL23;
+  base[le] = hi;
+  base[half + le] = lo;
+}
+
 template <bool SPLIT>
 __global__ __launch_bounds__(64 * kInferWaves) void k_sur_infer(int B, const float* __restrict__ X,
                                                                  const float* __restrict__ p,
+                                                                 const unsigned short* __restrict__ img,
                                                                  float* __restrict__ pred)
 {
   constexpr int kT = kSurH / 16;           // unit tiles
   constexpr int kS = kSurH / 32;           // k-steps of layers 2 and 3
   constexpr int kBlk = 64 * 8;             // bf16 per (tile, step) block
-  constexpr int kL1 = kT * kBlk;           // layer 1: one k-step
-  constexpr int kL23 = kT * kS * kBlk;
-  __shared__ __align__(16) unsigned short w1[2][kL1];
-  __shared__ __align__(16) unsigned short w23[2][2][kL23];  // [layer 2, 3][hi, lo]
+  __shared__ __align__(16) unsigned short wimg[kInferWords];
   __shared__ float bias[3][kSurH], w4[kSurH];
-  // weights split and reordered for the A operand
-  for (int e = threadIdx.x; e < kL1 + 2 * kL23; e += blockDim.x) {
-    const int layer = e < kL1 ? 0 : 1 + (e - kL1) / kL23;
-    const int le = layer == 0 ? e : (e - kL1) % kL23;
-    const int blk = le / kBlk, lane = (le % kBlk) >> 3, j = le & 7;
-    const int t = layer == 0 ? blk : blk / kS, s = layer == 0 ? 0 : blk % kS;
-    const int n = 16 * t + (lane & 15), g = lane >> 4;
-    const float w = layer == 0 ? p[kOffW1 + n * kSurIn + 8 * g + j]
-                               : p[(layer == 1 ? kOffW2 : kOffW3) + n * kSurH + infer_k(s, g, j)];
-    const unsigned short hi = bf16_bits(w);
-    const unsigned short lo = bf16_bits(w - bf16_value(hi));
-    if (layer == 0) {
-      w1[0][le] = hi;
-      w1[1][le] = lo;
-    } else {
-      w23[layer - 1][0][le] = hi;
-      w23[layer - 1][1][le] = lo;
-    }
+  const unsigned short* w1h = wimg;
+  const unsigned short* w1l = wimg + kInferL1;
+  // the prepared image, 16 bytes per load, all of a thread's loads in flight
+  {
+    constexpr int kChunks = kInferWords / 8;
+    static_assert(kChunks % (64 * kInferWaves) == 0, "whole chunks per thread");
+    constexpr int kPer = kChunks / (64 * kInferWaves);
+    uint4 v[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) v[k] = ((const uint4*)img)[k * 64 * kInferWaves + threadIdx.x];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) ((uint4*)wimg)[k * 64 * kInferWaves + threadIdx.x] = v[k];
   }
   for (int e = threadIdx.x; e < kSurH; e += blockDim.x) {
     bias[0][e] = p[kOffB1 + e];
@@ -239,8 +260,8 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_sur_infer(int B, const flo
       }
 #pragma unroll
       for (int t = 0; t < kT; ++t) {
-        const bf16x8 wa = __builtin_bit_cast(bf16x8, *(const u16x8*)&w1[0][t * kBlk + lane * 8]);
-        const bf16x8 wl = __builtin_bit_cast(bf16x8, *(const u16x8*)&w1[1][t * kBlk + lane * 8]);
+        const bf16x8 wa = __builtin_bit_cast(bf16x8, *(const u16x8*)&w1h[t * kBlk + lane * 8]);
+        const bf16x8 wl = __builtin_bit_cast(bf16x8, *(const u16x8*)&w1l[t * kBlk + lane * 8]);
 #pragma unroll
         for (int q = 0; q < kInferTiles; ++q) {
           f32x4 acc = {0.f, 0.f, 0.f, 0.f};
@@ -287,9 +308,9 @@ __global__ __launch_bounds__(64 * kInferWaves) void k_sur_infer(int B, const flo
         }
 #pragma unroll
         for (int t = 0; t < kT; ++t) {
-          const int off = (t * kS + s) * kBlk + lane * 8;
-          const bf16x8 wa = __builtin_bit_cast(bf16x8, *(const u16x8*)&w23[layer][0][off]);
-          const bf16x8 wl = __builtin_bit_cast(bf16x8, *(const u16x8*)&w23[layer][1][off]);
+          const int off = 2 * kInferL1 + 2 * layer * kInferL23 + (t * kS + s) * kBlk + lane * 8;
+          const bf16x8 wa = __builtin_bit_cast(bf16x8, *(const u16x8*)&wimg[off]);
+          const bf16x8 wl = __builtin_bit_cast(bf16x8, *(const u16x8*)&wimg[off + kInferL23]);
 #pragma unroll
           for (int q = 0; q < kInferTiles; ++q) {
             o[q][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa, bh[q], o[q][t], 0, 0, 0);
@@ -607,6 +628,7 @@ struct upc_sur {
   float* d_g = nullptr;   // [P] gradient of the current step
   float* d_act = nullptr; // Z1 H1 Z2 H2 Z3 H3 [6][max_batch][128]
   float* d_xb = nullptr;  // [max_batch][32]
+  unsigned short* d_img = nullptr;  // k_sur_infer's weight image of one network (prepared per call)
   float* d_tb = nullptr;
   float* d_pred = nullptr;
   float* d_gout = nullptr;
@@ -647,10 +669,11 @@ int infer(upc_sur* s, int net, const float* X, int B, float* pred, hipStream_t s
   (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
   const int per_blk = kInferWaves * kInferTiles;  // row tiles of a workgroup's pass
   const int blocks = std::max(1, std::min(ncu > 0 ? ncu : 256, nblk(((long)B + 15) / 16, per_blk)));
+  hipLaunchKernelGGL(k_sur_infer_prep, dim3(nblk(kInferL1 + 2 * kInferL23, 256)), dim3(256), 0, st, p, s->d_img);
   if (s->split)
-    hipLaunchKernelGGL(k_sur_infer<true>, dim3(blocks), dim3(64 * kInferWaves), 0, st, B, X, p, pred);
+    hipLaunchKernelGGL(k_sur_infer<true>, dim3(blocks), dim3(64 * kInferWaves), 0, st, B, X, p, s->d_img, pred);
   else
-    hipLaunchKernelGGL(k_sur_infer<false>, dim3(blocks), dim3(64 * kInferWaves), 0, st, B, X, p, pred);
+    hipLaunchKernelGGL(k_sur_infer<false>, dim3(blocks), dim3(64 * kInferWaves), 0, st, B, X, p, s->d_img, pred);
   UPC_HIP(hipGetLastError());
   return 0;
 }
@@ -723,7 +746,7 @@ int backward(upc_sur* s, int net, const float* X, int B, hipStream_t st)
 void free_sur(upc_sur* s)
 {
   void* ptrs[] = {s->d_p, s->d_m, s->d_v, s->d_g, s->d_act, s->d_xb, s->d_tb, s->d_pred, s->d_gout, s->d_loss,
-                  s->d_step};
+                  s->d_step, s->d_img};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (s->cap_stream) (void)hipStreamDestroy(s->cap_stream);
@@ -811,20 +834,8 @@ int lumi_grid_surrogate(upc_ctx* c, double* d_L, double* d_Ls, double* d_Lp, hip
     hipLaunchKernelGGL(k_sur_features, dim3(nblk(n, 256)), dim3(256), 0, st, sur_system(c->cfg), pts, pts + 1, n,
                        4, X, logL0);
     for (int net = 0; net < kSurNets; ++net) {
-      for (long o = 0; o < n; o += kSurMaxBatch) {
-        const int B = (int)std::min<long>(kSurMaxBatch, n - o);
-        const int Bp = (B + 31) / 32 * 32;
-        // the last chunk may be ragged: rows past n read the padded features buffer tail
-        if (Bp != B) {
-          UPC_HIP(hipMemcpyAsync(s->d_xb, X + o * kSurIn, sizeof(float) * kSurIn * B, hipMemcpyDeviceToDevice, st));
-          UPC_HIP(hipMemsetAsync(s->d_xb + (long)B * kSurIn, 0, sizeof(float) * kSurIn * (Bp - B), st));
-          rc = forward(s, net, s->d_xb, Bp, nullptr, s->d_pred, nullptr, st);
-          if (!rc) UPC_HIP(hipMemcpyAsync((net ? pb : pa) + o, s->d_pred, sizeof(float) * B, hipMemcpyDeviceToDevice, st));
-        } else {
-          rc = forward(s, net, X + o * kSurIn, B, nullptr, (net ? pb : pa) + o, nullptr, st);
-        }
-        if (rc) return rc;
-      }
+      rc = infer(s, net, X, (int)n, net ? pb : pa, st);  // any row count, one launch
+      if (rc) return rc;
     }
   }
   UPC_HIP(hipMemsetAsync(nf, 0, sizeof(unsigned long long), st));
@@ -902,6 +913,7 @@ int upc_sur_create(unsigned long long seed, upc_sur** out)
     UPC_HIP(hipMalloc(&s->d_g, sizeof(float) * kSurParams));
     UPC_HIP(hipMalloc(&s->d_act, sizeof(float) * 8L * kSurMaxBatch * kSurH));
     UPC_HIP(hipMalloc(&s->d_xb, sizeof(float) * (long)kSurMaxBatch * kSurIn));
+    UPC_HIP(hipMalloc(&s->d_img, sizeof(unsigned short) * kInferWords));
     UPC_HIP(hipMalloc(&s->d_tb, sizeof(float) * kSurMaxBatch));
     UPC_HIP(hipMalloc(&s->d_pred, sizeof(float) * kSurMaxBatch));
     UPC_HIP(hipMalloc(&s->d_gout, sizeof(float) * kSurMaxBatch));
