@@ -237,18 +237,22 @@ def other_configs():
     return out
 
 
-def surrogate_form(ctx, ref, steps=20000, fit_points=32768, fit_steps=15000, save_path=None):
+def surrogate_form(ctx, ref, steps=20000, fit_points=32768, fit_steps=15000, save_path=None, load_path=None):
     """The learned surrogate as a grid form.  The general network is trained
     on 2^20 FP64 quadrature points inside its domain (256 random systems x
     4096, `steps` Adam steps of 8192 rows, split-bf16 products) and checked
     on 2^16 held-out points of 32 new systems; then fine-tuned on
     `fit_points` quadrature points of this system and served through the
     FP64 anchor lattice with a 1e-3 bound.  Training happens before the
-    timed grids; its costs are reported beside them."""
+    timed grids; its costs are reported beside them.  load_path: a
+    system-fitted surrogate saved by an earlier run (--surrogate-out) is
+    served instead, without training (for profiling the grid form)."""
     import numpy as np
     import torch
     import upcgen_amd as U
     from upcgen_amd import surrogate as S
+    if load_path:
+        return surrogate_grid(ctx, ref, S.Surrogate.load(load_path), None)
     t0 = time.perf_counter()
     X, r, _, _ = S.make_dataset(256, 4096, seed=11)
     torch.cuda.synchronize()
@@ -266,8 +270,28 @@ def surrogate_form(ctx, ref, steps=20000, fit_points=32768, fit_steps=15000, sav
     pred, spread = s.log_lumi(Xh.contiguous(), L0h)
     err = (pred - lgh).abs().cpu().numpy()
     served_h = (s.err_scale * spread).cpu().numpy() <= 1e-3
+    f, t_label, t_fit = s.fit_system(ctx.config, n=fit_points, steps=fit_steps, lr=5e-4)
+    if save_path:
+        f.save(save_path)
+    training = {"rows": int(X.shape[0]), "steps": steps, "batch": 8192, "data_s": round(t_data, 2),
+                "train_s": round(t_train, 2),
+                "heldout_rows": int(Xh.shape[0]),
+                "heldout_abs_dlogL_median": float(np.median(err)),
+                "heldout_abs_dlogL_p99": float(np.quantile(err, 0.99)),
+                "heldout_spread_gate_served_frac": float(served_h.mean()),
+                "system_fit_points": fit_points, "system_fit_steps": fit_steps,
+                "system_fit_label_s": round(t_label, 2), "system_fit_train_s": round(t_fit, 2)}
+    return surrogate_grid(ctx, ref, f, training)
+
+
+def surrogate_grid(ctx, ref, f, training):
+    """Times the UPC_FORM_SURROGATE grid served by the system-fitted
+    surrogate f and checks every bin against the FP64 grid ref."""
+    import numpy as np
+    import torch
+    import upcgen_amd as U
+    from upcgen_amd import surrogate as S
     cfg = ctx.config
-    f, t_label, t_fit = s.fit_system(cfg, n=fit_points, steps=fit_steps, lr=5e-4)
     f.attach(ctx, max_err=1e-3)
     L = torch.empty_like(ref)
     ctx.lumi_grid(U.FORM_SURROGATE, out=L)  # warm
@@ -292,8 +316,6 @@ def surrogate_form(ctx, ref, steps=20000, fit_points=32768, fit_steps=15000, sav
     M = np.linspace(cfg.mmin, cfg.mmax, cfg.nm, endpoint=False)
     Y = np.linspace(cfg.ymin, cfg.ymax, cfg.ny, endpoint=False)
     inside = int((S.xi_max(cfg, M[:, None], Y[None, :]) <= S.XI_MAX).sum())
-    if save_path:
-        f.save(save_path)
     return {"ms_per_grid": round(ms, 3), "grid_points_per_s": round(POINTS / ms * 1e3, 1),
             "qags_ms": round(ph["qags"][0] / reps, 3),
             "contraction_ms": round((ph["lumi"][0] + ph["project"][0]) / reps, 3),
@@ -307,14 +329,7 @@ def surrogate_form(ctx, ref, steps=20000, fit_points=32768, fit_steps=15000, sav
             "gate": "FP64 anchors every 16 M x 4 Y bins; served where the network is within 5e-4 at the 4 "
                     "anchors around the bin; outside the domain (xi > 4) and elsewhere the FP64 grid of "
                     "those bins",
-            "training": {"rows": int(X.shape[0]), "steps": steps, "batch": 8192, "data_s": round(t_data, 2),
-                         "train_s": round(t_train, 2),
-                         "heldout_rows": int(Xh.shape[0]),
-                         "heldout_abs_dlogL_median": float(np.median(err)),
-                         "heldout_abs_dlogL_p99": float(np.quantile(err, 0.99)),
-                         "heldout_spread_gate_served_frac": float(served_h.mean()),
-                         "system_fit_points": fit_points, "system_fit_steps": fit_steps,
-                         "system_fit_label_s": round(t_label, 2), "system_fit_train_s": round(t_fit, 2)}}
+            "training": training if training else "loaded (no training in this run)"}
 
 
 def main():
@@ -334,6 +349,8 @@ def main():
     ap.add_argument("--no-surrogate", action="store_true",
                     help="skip the surrogate form (training it takes about a minute)")
     ap.add_argument("--surrogate-out", default=None, help="save the system-fitted surrogate (.npz)")
+    ap.add_argument("--surrogate-in", default=None,
+                    help="serve a saved system-fitted surrogate (.npz from --surrogate-out) instead of training")
     ap.add_argument("--no-other-configs", action="store_true",
                     help="skip the 101x41 point-flux grid and the 10^8 ditau event draws")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -492,7 +509,8 @@ def main():
             if f in (U.FORM_MFMA, U.FORM_MFMA_FP32):
                 compare[name]["fp64_guard_items"] = ctx.info.mfma_fp64_items
         if not args.no_surrogate:
-            compare["surrogate"] = surrogate_form(ctx, ref, save_path=args.surrogate_out)
+            compare["surrogate"] = surrogate_form(ctx, ref, save_path=args.surrogate_out,
+                                                  load_path=args.surrogate_in)
         for name, v in compare.items():
             v["speedup_vs_fp64_grid"] = round(compare["fp64"]["ms_per_grid"] / v["ms_per_grid"], 3)
     out = {
