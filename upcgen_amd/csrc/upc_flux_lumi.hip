@@ -53,14 +53,17 @@ __global__ void k_rows(const double* __restrict__ pts, int npts, int nrows, Flux
 
 // Point-charge flux for nodes with b > 2R (or FLUX_POINT 1); QAGS task ids
 // for the others.
+// row_need (may be null): rows that no needed bin reads are skipped.
 __global__ void k_flux_point_rows(int nrows, FluxParams fp, const double* __restrict__ row_k,
                                   const double* __restrict__ row_logd, const int* __restrict__ row_cnt,
                                   const long long* __restrict__ row_off, double* __restrict__ flux,
-                                  TaskRec* __restrict__ recs)
+                                  TaskRec* __restrict__ recs, const unsigned char* __restrict__ row_need)
 {
   const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= (long)nrows * kNB1) return;
-  point_row_node(gid / kNB1, (int)(gid % kNB1), fp, row_k, row_logd, row_cnt, row_off, flux, recs);
+  const long r = gid / kNB1;
+  if (row_need && !row_need[r]) return;
+  point_row_node(r, (int)(gid % kNB1), fp, row_k, row_logd, row_cnt, row_off, flux, recs);
 }
 
 // ---------------------------------------------------------------------------
@@ -149,7 +152,7 @@ static int run_flux(upc_ctx* c, int npts, int mirror_ny, hipStream_t s, const un
   hipLaunchKernelGGL(k_scan_apply<>, dim3(nparts), dim3(kScanBlock), 0, s, w.d_row_cnt, (long)nrows, w.d_part,
                      w.d_row_off);
   hipLaunchKernelGGL(k_flux_point_rows, dim3(nblk((long)nrows * kNB1, 256)), dim3(256), 0, s, nrows, fp,
-                     w.d_row_k, w.d_row_logd, w.d_row_cnt, w.d_row_off, w.d_flux, (TaskRec*)w.d_tasks);
+                     w.d_row_k, w.d_row_logd, w.d_row_cnt, w.d_row_off, w.d_flux, (TaskRec*)w.d_tasks, row_need);
   UPC_HIP(hipGetLastError());
   }
   if (!c->cfg.flux_point) {

@@ -540,18 +540,6 @@ struct AnchorGrid {
   int nm, ny, em, ey, nam, nay;
 };
 
-// (M, Y) of the anchors: M at MY[a], Y at MY[na + a].
-__global__ void k_sur_anchor_points(AnchorGrid g, const double* __restrict__ pts, double* __restrict__ MY)
-{
-  const int a = blockIdx.x * blockDim.x + threadIdx.x;
-  const int na = g.nam * g.nay;
-  if (a >= na) return;
-  const int am = a / g.nay, ay = a - am * g.nay;
-  const long p = (long)anchor_pos(am, g.nm, g.em) * g.ny + anchor_pos(ay, g.ny, g.ey);
-  MY[a] = pts[4 * p];
-  MY[na + a] = pts[4 * p + 1];
-}
-
 // The network's |d log L| at each anchor against the FP64 quadrature value
 // La there (infinite where the anchor lies outside the domain or either
 // value is not finite).
@@ -568,23 +556,49 @@ __global__ void k_sur_anchor_err(AnchorGrid g, const float* __restrict__ pa, con
   aerr[a] = isfinite(lg) && la > 0.0 && isfinite(la) ? fabs(lg - log(la)) : INFINITY;
 }
 
-// Grid gate with anchors: an anchor bin takes its quadrature value; any
-// other bin takes the network's value when the largest anchor error at the
-// four corners of its anchor cell is within margin * max_err, and falls
-// back to the quadrature otherwise.
+// Bins that the FP64 quadrature answers before the gate: the anchors and
+// the bins outside the network's domain (log L0 not finite).  *count
+// receives their number.
+__global__ void k_sur_need_before(AnchorGrid g, const double* __restrict__ logL0, unsigned char* __restrict__ need,
+                                  unsigned long long* __restrict__ count)
+{
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long)g.nm * g.ny) return;
+  const int im = (int)(i / g.ny), iy = (int)(i - (long)im * g.ny);
+  const bool anchor = anchor_of(im, g.nm, g.em, g.nam) >= 0 && anchor_of(iy, g.ny, g.ey, g.nay) >= 0;
+  const bool v = anchor || !isfinite(logL0[i]);
+  need[i] = v;
+  if (v) atomicAdd(count, 1ull);
+}
+
+// L at each anchor from the FP64 grid values (stored x dm dy).
+__global__ void k_sur_anchor_vals(AnchorGrid g, const double* __restrict__ Lq, const double* __restrict__ pts,
+                                  double* __restrict__ La)
+{
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= g.nam * g.nay) return;
+  const int am = a / g.nay, ay = a - am * g.nay;
+  const long p = (long)anchor_pos(am, g.nm, g.em) * g.ny + anchor_pos(ay, g.ny, g.ey);
+  La[a] = Lq[p] / (pts[4 * p + 2] * pts[4 * p + 3]);
+}
+
+// Grid gate with anchors: a bin answered before the gate (need_before: an
+// anchor, or outside the domain) takes its quadrature value Lq; any other
+// bin takes the network's value when the largest anchor error at the four
+// corners of its anchor cell is within margin * max_err, and falls back to
+// the quadrature otherwise.
 __global__ void k_sur_gate(AnchorGrid g, const float* __restrict__ pa, const float* __restrict__ pb,
                            const double* __restrict__ logL0, double mu, double sigma,
-                           const double* __restrict__ aerr, const double* __restrict__ La, double bound,
-                           const double* __restrict__ pts, double* __restrict__ L, int* __restrict__ fallback,
-                           unsigned long long* __restrict__ nfall)
+                           const double* __restrict__ aerr, const unsigned char* __restrict__ need_before,
+                           const double* __restrict__ Lq, double bound, const double* __restrict__ pts,
+                           double* __restrict__ L, int* __restrict__ fallback, unsigned long long* __restrict__ nfall)
 {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (long)g.nm * g.ny) return;
   const int im = (int)(i / g.ny), iy = (int)(i - (long)im * g.ny);
   const double dmdy = pts[4 * i + 2] * pts[4 * i + 3];
-  const int am = anchor_of(im, g.nm, g.em, g.nam), ay = anchor_of(iy, g.ny, g.ey, g.nay);
-  if (am >= 0 && ay >= 0) {
-    L[i] = La[am * g.nay + ay] * dmdy;
+  if (need_before[i]) {
+    L[i] = Lq[i];
     return;
   }
   const int m0 = im / g.em, m1 = m0 + 1 < g.nam ? m0 + 1 : g.nam - 1;
@@ -817,14 +831,14 @@ int lumi_grid_surrogate(upc_ctx* c, double* d_L, double* d_Ls, double* d_Lp, hip
   }
   const int na = ag.nam * ag.nay;
   float *X = nullptr, *pa = nullptr, *pb = nullptr;
-  double *logL0 = nullptr, *MY = nullptr, *Lf = nullptr, *pts = nullptr, *La = nullptr, *aerr = nullptr;
+  double *logL0 = nullptr, *Lf = nullptr, *pts = nullptr, *La = nullptr, *aerr = nullptr;
   int* idx = nullptr;
   unsigned long long* nf = nullptr;
   UPC_HIP(hipMallocAsync((void**)&X, sizeof(float) * kSurIn * n, st));
   UPC_HIP(hipMallocAsync((void**)&pa, sizeof(float) * 2 * n, st));
   UPC_HIP(hipMallocAsync((void**)&logL0, sizeof(double) * n, st));
   UPC_HIP(hipMallocAsync((void**)&idx, sizeof(int) * n, st));
-  UPC_HIP(hipMallocAsync((void**)&nf, sizeof(unsigned long long), st));
+  UPC_HIP(hipMallocAsync((void**)&nf, 2 * sizeof(unsigned long long), st));
   // the point calls below reuse the work buffer: keep the grid's (M, Y, dm, dy)
   UPC_HIP(hipMallocAsync((void**)&pts, sizeof(double) * 4 * n, st));
   UPC_HIP(hipMemcpyAsync(pts, c->work.d_points, sizeof(double) * 4 * n, hipMemcpyDeviceToDevice, st));
@@ -838,31 +852,40 @@ int lumi_grid_surrogate(upc_ctx* c, double* d_L, double* d_Ls, double* d_Lp, hip
       if (rc) return rc;
     }
   }
-  UPC_HIP(hipMemsetAsync(nf, 0, sizeof(unsigned long long), st));
+  UPC_HIP(hipMemsetAsync(nf, 0, 2 * sizeof(unsigned long long), st));
+  unsigned char* need_before = nullptr;
+  double* Lq = nullptr;
   if (anchored) {
-    // the network judged by the FP64 quadrature on the anchor lattice
-    UPC_HIP(hipMallocAsync((void**)&MY, sizeof(double) * 2 * na, st));
+    // The network judged by the FP64 quadrature on the anchor lattice.  The
+    // bins outside its domain go to the quadrature whatever the network
+    // says, so their flux rows run in the same QAGS launch as the anchors'
+    // (the anchors alone are too few integrals to fill the GPU); only the
+    // bins that the gate then rejects need a second launch.
     UPC_HIP(hipMallocAsync((void**)&La, sizeof(double) * na, st));
     UPC_HIP(hipMallocAsync((void**)&aerr, sizeof(double) * na, st));
-    hipLaunchKernelGGL(k_sur_anchor_points, dim3(nblk(na, 256)), dim3(256), 0, st, ag, pts, MY);
-    rc = lumi_points_fp64(c, MY, na, La, nullptr, nullptr, st);
+    UPC_HIP(hipMallocAsync((void**)&need_before, (size_t)n, st));
+    UPC_HIP(hipMallocAsync((void**)&Lq, sizeof(double) * n, st));
+    hipLaunchKernelGGL(k_sur_need_before, dim3(nblk(n, 256)), dim3(256), 0, st, ag, logL0, need_before, nf + 1);
+    rc = lumi_grid_subset_fp64(c, need_before, Lq, st);
     if (rc) return rc;
     PhaseScope ph(c, st, UPC_PHASE_LUMI);
+    hipLaunchKernelGGL(k_sur_anchor_vals, dim3(nblk(na, 256)), dim3(256), 0, st, ag, Lq, pts, La);
     hipLaunchKernelGGL(k_sur_anchor_err, dim3(nblk(na, 256)), dim3(256), 0, st, ag, pa, pb, logL0, s->mu,
                        s->sigma, La, aerr);
     hipLaunchKernelGGL(k_sur_gate, dim3(nblk(n, 256)), dim3(256), 0, st, ag, pa, pb, logL0, s->mu, s->sigma, aerr,
-                       La, kAnchorMargin * at->max_err, pts, d_L, idx, nf);
+                       need_before, Lq, kAnchorMargin * at->max_err, pts, d_L, idx, nf);
   } else {
     PhaseScope ph(c, st, UPC_PHASE_LUMI);
     hipLaunchKernelGGL(k_sur_combine, dim3(nblk(n, 256)), dim3(256), 0, st, n, pa, pb, logL0, s->mu, s->sigma,
                        s->err_scale, at->max_err, pts, d_L, idx, nf, (double*)nullptr);
   }
-  unsigned long long h_nf = 0;
-  UPC_HIP(hipMemcpyAsync(&h_nf, nf, sizeof(h_nf), hipMemcpyDeviceToHost, st));
+  unsigned long long h_cnt[2] = {0, 0};  // bins rejected by the gate, bins answered before it
+  UPC_HIP(hipMemcpyAsync(h_cnt, nf, sizeof(h_cnt), hipMemcpyDeviceToHost, st));
   UPC_HIP(hipStreamSynchronize(st));
+  const unsigned long long h_nf = h_cnt[0];
   at->anchors = na;
-  at->served = n - na - (long long)h_nf;
-  at->fallback = (long long)h_nf + na;
+  at->served = n - (long long)h_cnt[1] - (long long)h_nf;
+  at->fallback = (long long)h_nf + (long long)h_cnt[1];
   if (h_nf) {
     // the fallback bins by the FP64 grid restricted to them: their flux rows
     // (shared between mirror bins as in the full grid) and their
@@ -878,7 +901,7 @@ int lumi_grid_surrogate(upc_ctx* c, double* d_L, double* d_Ls, double* d_Lp, hip
     (void)hipFreeAsync(need, st);
     (void)hipFreeAsync(Lf, st);
   }
-  void* bufs[] = {X, pa, logL0, idx, nf, pts, MY, La, aerr};
+  void* bufs[] = {X, pa, logL0, idx, nf, pts, La, aerr, need_before, Lq};
   for (void* b : bufs)
     if (b) (void)hipFreeAsync(b, st);
   UPC_HIP(hipGetLastError());
