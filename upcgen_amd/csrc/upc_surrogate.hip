@@ -839,7 +839,7 @@ int lumi_grid_surrogate(upc_ctx* c, double* d_L, double* d_Ls, double* d_Lp, hip
   UPC_HIP(hipMallocAsync((void**)&logL0, sizeof(double) * n, st));
   UPC_HIP(hipMallocAsync((void**)&idx, sizeof(int) * n, st));
   UPC_HIP(hipMallocAsync((void**)&nf, 2 * sizeof(unsigned long long), st));
-  // the point calls below reuse the work buffer: keep the grid's (M, Y, dm, dy)
+  // the subset grids below rebuild (and may reallocate) the work buffer: keep the grid's (M, Y, dm, dy)
   UPC_HIP(hipMallocAsync((void**)&pts, sizeof(double) * 4 * n, st));
   UPC_HIP(hipMemcpyAsync(pts, c->work.d_points, sizeof(double) * 4 * n, hipMemcpyDeviceToDevice, st));
   pb = pa + n;
